@@ -1,0 +1,426 @@
+// C ABI of the MI355X ir_interpolate_at path (declarations in include/brille_amd.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <vector>
+
+#include "../../include/brille_amd.h"
+#include "grid_state.hpp"
+#include "interp_kernels.hpp"
+
+using namespace brille_amd;
+
+namespace brille_amd {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                              \
+  do {                                                                                             \
+    hipError_t e_ = (expr);                                                                        \
+    if (e_ != hipSuccess) return fail(BRILLE_ERR_DEVICE, std::string(#expr ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+template <typename T>
+static int upload(const T* host, size_t n, const T** dev, std::vector<void*>& allocs) {
+  if (n == 0 || host == nullptr) {
+    *dev = nullptr;
+    return BRILLE_OK;
+  }
+  void* p = nullptr;
+  HIP_TRY(hipMalloc(&p, n * sizeof(T)));
+  allocs.push_back(p);
+  HIP_TRY(hipMemcpy(p, host, n * sizeof(T), hipMemcpyHostToDevice));
+  *dev = static_cast<const T*>(p);
+  return BRILLE_OK;
+}
+
+static void free_all(std::vector<void*>& allocs) {
+  for (void* p : allocs) (void)hipFree(p);
+  allocs.clear();
+}
+
+static void make_tols(double Ttol, int tol, double* rel, double* abs_) {
+  // approx_float::tols  src/approx_float.hpp:79-100
+  double Trel = std::numeric_limits<double>::epsilon();
+  double Tabs = 5.0 / 1000000000000000.0;
+  Trel *= static_cast<double>(tol) * 10000.0;
+  if (Ttol > Trel) Trel = Ttol;
+  if (Ttol > Tabs) Tabs = Ttol;
+  *rel = Trel;
+  *abs_ = Tabs;
+}
+
+__global__ void k_err_init(ErrRec* e) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    e->first_bad = ~0ull;
+    e->count = 0ull;
+  }
+}
+
+template <typename Real, bool Fast>
+static void launch_interp(const brille_grid* G, const LaunchArgs& a, hipStream_t s) {
+  const uint64_t blocks = (a.n + TILE - 1) / TILE;
+  if (blocks == 0) return;
+  hipLaunchKernelGGL((k_ir_interpolate<Real, Fast>), dim3((unsigned)blocks), dim3(TILE), 0, s, G->g, G->d, a);
+}
+
+// stage (a) only
+__global__ __launch_bounds__(256) void k_ir_moveinto(GridConst g, const double* Q, uint64_t n, double* q_out,
+                                                     int32_t* tau_out, uint32_t* rot_out, uint32_t* invrot_out,
+                                                     uint8_t* status_out, ErrRec* err, uint64_t base) {
+  const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const Tol tbz{g.tol_bz_rel, g.tol_bz_abs};
+  const Tol tdef{g.tol_def_rel, g.tol_def_abs};
+  double Qp[3] = {Q[3 * p], Q[3 * p + 1], Q[3 * p + 2]};
+  double q[3] = {0, 0, 0};
+  int32_t tau[3] = {0, 0, 0};
+  uint32_t r = g.identity, ir = g.identity;
+  int st = moveinto_exact(g, Qp, q, tau, tbz);
+  if (st == BRILLE_PT_OK) st = wedge_search_exact(g, q, &r, &ir, tbz, tdef);
+  if (q_out) { q_out[3 * p] = q[0]; q_out[3 * p + 1] = q[1]; q_out[3 * p + 2] = q[2]; }
+  if (tau_out) { tau_out[3 * p] = tau[0]; tau_out[3 * p + 1] = tau[1]; tau_out[3 * p + 2] = tau[2]; }
+  if (rot_out) rot_out[p] = r;
+  if (invrot_out) invrot_out[p] = ir;
+  if (status_out) status_out[p] = (uint8_t)st;
+  if (st != BRILLE_PT_OK) {
+    atomicMin(&err->first_bad, ((unsigned long long)(base + p) << 3) | (unsigned long long)st);
+    atomicAdd(&err->count, 1ull);
+  }
+}
+
+static std::string fmt_q(const double* q) {
+  char buf[160];
+  std::snprintf(buf, sizeof(buf), "[%.17g, %.17g, %.17g]", q[0], q[1], q[2]);
+  return buf;
+}
+
+// Translate the first failing point into the reference's error text.
+static int report_point_error(const ErrRec& e, const double* Qhost_row) {
+  const uint64_t idx = e.first_bad >> 3;
+  const int st = static_cast<int>(e.first_bad & 7ull);
+  std::string q = Qhost_row ? fmt_q(Qhost_row) : ("#" + std::to_string(idx));
+  switch (st) {
+    case BRILLE_PT_OUTSIDE_BZ:
+      return fail(BRILLE_ERR_OUTSIDE_BZ, "Not all points inside Brillouin zone (first: Q = " + q + ")");
+    case BRILLE_PT_OUTSIDE_IR:
+      return fail(BRILLE_ERR_OUTSIDE_IR, "Q = " + q + " is outside of the irreducible BrillouinZone");
+    case BRILLE_PT_NULL_NODE:
+      return fail(BRILLE_ERR_NULL_NODE, "Attempting to access null node for point " + q);
+    default: {
+      std::string m = "interpolate_at failed to find " + std::to_string(e.count) + " point";
+      m += e.count > 1 ? "s." : ".";
+      return fail(BRILLE_ERR_NOT_FOUND, m);
+    }
+  }
+}
+
+}  // namespace brille_amd
+
+// ===========================================================================
+extern "C" {
+
+int brille_abi_version(void) { return BRILLE_AMD_ABI_VERSION; }
+
+int brille_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+size_t brille_last_error(char* buffer, size_t length) {
+  const std::string& m = g_last_error;
+  if (buffer && length) {
+    size_t k = std::min(length - 1, m.size());
+    std::memcpy(buffer, m.data(), k);
+    buffer[k] = '\0';
+  }
+  return m.size();
+}
+
+int brille_grid_create(const brille_grid_desc* desc, brille_grid** out) {
+  if (!desc || !out) return fail(BRILLE_ERR_INVALID_ARGUMENT, "null argument");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(BRILLE_ERR_NO_DEVICE, "no HIP device available");
+  if (desc->n_faces < 4 || desc->n_faces > kMaxFaces)
+    return fail(BRILLE_ERR_INVALID_ARGUMENT, "first Brillouin zone must have between 4 and 32 faces");
+  if (desc->n_ops < 1 || desc->identity_index >= desc->n_ops)
+    return fail(BRILLE_ERR_INVALID_ARGUMENT, "point group must contain the identity");
+  auto* G = new brille_grid();
+  GridConst& g = G->g;
+  std::memset(&g, 0, sizeof(g));
+  std::memcpy(g.outer_to_xyz, desc->outer_to_xyz, sizeof(g.outer_to_xyz));
+  std::memcpy(g.outer_Gr, desc->outer_metric_recip, sizeof(g.outer_Gr));
+  std::memcpy(g.outer_Gd, desc->outer_metric_real, sizeof(g.outer_Gd));
+  g.outer_scale = desc->outer_cross_scale;
+  std::memcpy(g.prim_Gr, desc->prim_metric_recip, sizeof(g.prim_Gr));
+  std::memcpy(g.prim_Gd, desc->prim_metric_real, sizeof(g.prim_Gd));
+  g.prim_scale = desc->prim_cross_scale;
+  std::memcpy(g.P6t, desc->to_primitive_6Pt, sizeof(g.P6t));
+  std::memcpy(g.invPt, desc->from_primitive_invPt, sizeof(g.invPt));
+  g.transform_needed = desc->transform_needed;
+  g.no_ir_mirroring = desc->no_ir_mirroring;
+  g.has_seed = desc->has_wedge_seed;
+  g.F = desc->n_faces;
+  g.W = desc->n_wedge;
+  g.G = desc->n_ops;
+  g.identity = desc->identity_index;
+  make_tols(desc->float_tolerance, desc->approx_digits, &g.tol_bz_rel, &g.tol_bz_abs);
+  make_tols(0.0, 1, &g.tol_def_rel, &g.tol_def_abs);
+  auto& A = G->grid_allocs;
+  const size_t F3 = 3 * (size_t)desc->n_faces;
+  int rc = BRILLE_OK;
+  auto chk = [&](int r) { if (r != BRILLE_OK && rc == BRILLE_OK) rc = r; };
+  chk(upload(desc->prim_plane_a, F3, &g.prim_pa, A));
+  chk(upload(desc->prim_plane_b, F3, &g.prim_pb, A));
+  chk(upload(desc->prim_plane_c, F3, &g.prim_pc, A));
+  chk(upload(desc->prim_normals, F3, &g.prim_n, A));
+  chk(upload(desc->prim_taus, F3, &g.prim_taus, A));
+  chk(upload(desc->prim_tau_lens, (size_t)desc->n_faces, &g.prim_tau_lens, A));
+  chk(upload(desc->outer_plane_a, F3, &g.outer_pa, A));
+  chk(upload(desc->outer_plane_b, F3, &g.outer_pb, A));
+  chk(upload(desc->outer_plane_c, F3, &g.outer_pc, A));
+  chk(upload(desc->wedge_normals, 3 * (size_t)desc->n_wedge, &g.wedge_n, A));
+  chk(upload(desc->rotations, 9 * (size_t)desc->n_ops, &g.rot, A));
+  chk(upload(desc->inverse_index, (size_t)desc->n_ops, &g.inv_index, A));
+  // trellis (may be empty for a zone-only handle)
+  uint64_t nodes = 1;
+  for (int d = 0; d < 3; ++d) {
+    g.nk[d] = desc->n_knots[d];
+    if (desc->n_knots[d] >= 2) {
+      chk(upload(desc->knots[d], (size_t)desc->n_knots[d], &g.knots[d], A));
+      g.knot_lo[d] = desc->knots[d][0];
+      double step = (desc->knots[d][desc->n_knots[d] - 1] - desc->knots[d][0]) / (double)(desc->n_knots[d] - 1);
+      g.knot_inv_step[d] = step > 0 ? 1.0 / step : 0.0;
+      nodes *= desc->n_knots[d] - 1;
+    } else {
+      nodes = 0;
+    }
+  }
+  if (desc->n_nodes && nodes != desc->n_nodes) {
+    free_all(A);
+    delete G;
+    return fail(BRILLE_ERR_INVALID_ARGUMENT, "node count does not match the knots");
+  }
+  g.n_nodes = desc->n_nodes;
+  chk(upload(desc->node_type, (size_t)desc->n_nodes, &g.node_type, A));
+  chk(upload(desc->node_index, (size_t)desc->n_nodes, &g.node_index, A));
+  chk(upload(desc->cube_vertices, 8 * (size_t)desc->n_cubes, &g.cube_v, A));
+  if (desc->n_polys) chk(upload(desc->poly_tet_offset, (size_t)desc->n_polys + 1, &g.poly_off, A));
+  chk(upload(desc->tet_vertices, 4 * (size_t)desc->n_tets, &g.tet_v, A));
+  chk(upload(desc->tet_circumsphere, 4 * (size_t)desc->n_tets, &g.tet_c, A));
+  chk(upload(desc->tet_volume, (size_t)desc->n_tets, &g.tet_vol, A));
+  chk(upload(desc->vertices, 3 * (size_t)desc->n_vertices, &g.verts, A));
+  if (rc == BRILLE_OK) rc = build_fast_tables(desc, G);
+  if (rc != BRILLE_OK) {
+    free_all(A);
+    delete G;
+    return rc;
+  }
+  G->n_vertices = desc->n_vertices;
+  G->n_polys = desc->n_polys;
+  G->n_tets = desc->n_tets;
+  void* e = nullptr;
+  if (hipMalloc(&e, sizeof(ErrRec)) != hipSuccess) {
+    free_all(A);
+    delete G;
+    return fail(BRILLE_ERR_DEVICE, "hipMalloc failed for the error record");
+  }
+  G->err = static_cast<ErrRec*>(e);
+  G->host_desc_poly_off.assign(desc->poly_tet_offset, desc->poly_tet_offset + (desc->n_polys ? desc->n_polys + 1 : 0));
+  *out = G;
+  return BRILLE_OK;
+}
+
+int brille_grid_destroy(brille_grid* G) {
+  if (!G) return BRILLE_OK;
+  (void)hipDeviceSynchronize();
+  free_all(G->grid_allocs);
+  free_all(G->data_allocs);
+  free_proposal(G);
+  free_staging(G);
+  if (G->err) (void)hipFree(G->err);
+  delete G;
+  return BRILLE_OK;
+}
+
+int brille_grid_fill(brille_grid* G, const brille_data_desc* vals, const brille_data_desc* vecs,
+                     const brille_gamma_desc* gamma, const brille_perm_desc* perm) {
+  if (!G || !vals || !vecs) return fail(BRILLE_ERR_INVALID_ARGUMENT, "null argument");
+  if (vals->n_vertices != G->n_vertices || vecs->n_vertices != G->n_vertices)
+    return fail(BRILLE_ERR_INVALID_ARGUMENT, "Provided " + std::to_string(vals->n_vertices) + " arrays but " +
+                                                  std::to_string(G->n_vertices) + " were expected!");
+  if (vals->branches != vecs->branches || vals->branches == 0)
+    return fail(BRILLE_ERR_INVALID_ARGUMENT, "Inconsistent values and vectors provided to DualInterpolator");
+  if (vals->is_complex || !vecs->is_complex)
+    return fail(BRILLE_ERR_INVALID_ARGUMENT, "values must be real and vectors complex");
+  for (const brille_data_desc* x : {vals, vecs}) {
+    if (x->elements[1] % 3) return fail(BRILLE_ERR_INVALID_ARGUMENT, "Vectors must have 3N elements per branch");
+    if (x->elements[2] % 9) return fail(BRILLE_ERR_INVALID_ARGUMENT, "Matrices must have 9N elements per branch");
+    if (x->elements[0] + x->elements[1] + x->elements[2] == 0)
+      return fail(BRILLE_ERR_INVALID_ARGUMENT, "the elements per branch must not all be zero");
+  }
+  free_all(G->data_allocs);
+  DataConst& d = G->d;
+  std::memset(&d, 0, sizeof(d));
+  d.V = G->n_vertices;
+  d.B = vals->branches;
+  d.sv = vals->elements[0] + vals->elements[1] + vals->elements[2];
+  d.sc = vecs->elements[0] + vecs->elements[1] + vecs->elements[2];
+  for (int i = 0; i < 3; ++i) { d.ev[i] = vals->elements[i]; d.ec[i] = vecs->elements[i]; }
+  d.val_rot = vals->rotates_like;
+  d.val_lu = vals->length_unit;
+  d.vec_rot = vecs->rotates_like;
+  d.vec_lu = vecs->length_unit;
+  auto& A = G->data_allocs;
+  const size_t nv = (size_t)d.V * d.B * d.sv, nc = (size_t)d.V * d.B * d.sc;
+  int rc = BRILLE_OK;
+  auto chk = [&](int r) { if (r != BRILLE_OK && rc == BRILLE_OK) rc = r; };
+  chk(upload(vals->data, nv, &d.vals64, A));
+  chk(upload(reinterpret_cast<const double2*>(vecs->data), nc, &d.vecs64, A));
+  std::vector<float> f32(nv);
+  for (size_t i = 0; i < nv; ++i) f32[i] = static_cast<float>(vals->data[i]);
+  chk(upload(f32.data(), nv, &d.vals32, A));
+  std::vector<float2> c32(nc);
+  for (size_t i = 0; i < nc; ++i) c32[i] = make_float2((float)vecs->data[2 * i], (float)vecs->data[2 * i + 1]);
+  chk(upload(c32.data(), nc, &d.vecs32, A));
+  if (gamma) {
+    if (gamma->n_ops != G->g.G) rc = fail(BRILLE_ERR_INVALID_ARGUMENT, "GammaTable and point group sizes differ");
+    d.n_atoms = gamma->n_atoms;
+    d.gamma_ops = gamma->n_ops;
+    chk(upload(gamma->F0, (size_t)gamma->n_atoms * gamma->n_ops, &d.F0, A));
+    chk(upload(gamma->vector_index, (size_t)gamma->n_atoms * gamma->n_ops, &d.vidx, A));
+    chk(upload(gamma->vectors, 3 * (size_t)gamma->n_vectors, &d.gvec, A));
+    if (gamma->cart_rotations) chk(upload(gamma->cart_rotations, 9 * (size_t)gamma->n_ops, &d.cart_rot, A));
+  }
+  if (perm && perm->n_keys) {
+    d.n_keys = perm->n_keys;
+    chk(upload(perm->keys, (size_t)perm->n_keys, &d.keys, A));
+    chk(upload(perm->perm_index, (size_t)perm->n_keys, &d.perm_index, A));
+    chk(upload(perm->perms, (size_t)perm->n_perms * d.B, &d.perms, A));
+  }
+  if (rc != BRILLE_OK) {
+    free_all(A);
+    G->filled = false;
+    return rc;
+  }
+  G->filled = true;
+  return BRILLE_OK;
+}
+
+uint64_t brille_bytes_per_point(const brille_grid* G, int32_t precision) {
+  if (!G || !G->filled) return 0;
+  const uint64_t s = precision == BRILLE_FP32 ? 4 : 8;
+  return (uint64_t)G->d.B * G->d.sv * s + (uint64_t)G->d.B * G->d.sc * 2 * s;
+}
+
+int brille_ir_interpolate_at(brille_grid* G, const double* Q, uint64_t n, const brille_run_opts* opts_in,
+                             void* values_out, void* vectors_out, const brille_aux_out* aux, void* stream) {
+  return run_interpolate(G, Q, n, opts_in, values_out, vectors_out, aux, static_cast<hipStream_t>(stream), nullptr, 0);
+}
+
+int brille_ir_moveinto(brille_grid* G, const double* Q, uint64_t n, int32_t memory, double* q_out, int32_t* tau_out,
+                       uint32_t* rot_out, uint32_t* invrot_out, uint8_t* status_out, void* stream_v) {
+  if (!G || (!Q && n)) return fail(BRILLE_ERR_INVALID_ARGUMENT, "null argument");
+  if (n == 0) return BRILLE_OK;
+  hipStream_t s = static_cast<hipStream_t>(stream_v);
+  const double* dQ = Q;
+  double* dq = q_out;
+  int32_t* dtau = tau_out;
+  uint32_t *drot = rot_out, *dinv = invrot_out;
+  uint8_t* dst = status_out;
+  std::vector<void*> tmp;
+  auto dalloc = [&](void** p, size_t bytes) -> int {
+    HIP_TRY(hipMalloc(p, bytes));
+    tmp.push_back(*p);
+    return BRILLE_OK;
+  };
+  int rc = BRILLE_OK;
+  if (memory == BRILLE_MEM_HOST) {
+    void* p;
+    if ((rc = dalloc(&p, n * 3 * sizeof(double))) != BRILLE_OK) { free_all(tmp); return rc; }
+    HIP_TRY(hipMemcpy(p, Q, n * 3 * sizeof(double), hipMemcpyHostToDevice));
+    dQ = static_cast<const double*>(p);
+    if (q_out) { if ((rc = dalloc(&p, n * 3 * sizeof(double))) != BRILLE_OK) { free_all(tmp); return rc; } dq = (double*)p; }
+    if (tau_out) { if ((rc = dalloc(&p, n * 3 * sizeof(int32_t))) != BRILLE_OK) { free_all(tmp); return rc; } dtau = (int32_t*)p; }
+    if (rot_out) { if ((rc = dalloc(&p, n * sizeof(uint32_t))) != BRILLE_OK) { free_all(tmp); return rc; } drot = (uint32_t*)p; }
+    if (invrot_out) { if ((rc = dalloc(&p, n * sizeof(uint32_t))) != BRILLE_OK) { free_all(tmp); return rc; } dinv = (uint32_t*)p; }
+    if (status_out) { if ((rc = dalloc(&p, n)) != BRILLE_OK) { free_all(tmp); return rc; } dst = (uint8_t*)p; }
+  }
+  hipLaunchKernelGGL(k_err_init, dim3(1), dim3(1), 0, s, G->err);
+  const uint64_t blocks = (n + 255) / 256;
+  hipLaunchKernelGGL(k_ir_moveinto, dim3((unsigned)blocks), dim3(256), 0, s, G->g, dQ, n, dq, dtau, drot, dinv, dst, G->err,
+                     (uint64_t)0);
+  HIP_TRY(hipGetLastError());
+  ErrRec e;
+  HIP_TRY(hipMemcpyAsync(&e, G->err, sizeof(e), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  if (memory == BRILLE_MEM_HOST) {
+    if (q_out) HIP_TRY(hipMemcpy(q_out, dq, n * 3 * sizeof(double), hipMemcpyDeviceToHost));
+    if (tau_out) HIP_TRY(hipMemcpy(tau_out, dtau, n * 3 * sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (rot_out) HIP_TRY(hipMemcpy(rot_out, drot, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    if (invrot_out) HIP_TRY(hipMemcpy(invrot_out, dinv, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    if (status_out) HIP_TRY(hipMemcpy(status_out, dst, n, hipMemcpyDeviceToHost));
+    free_all(tmp);
+  }
+  if (e.count) {
+    double row[3];
+    const uint64_t idx = e.first_bad >> 3;
+    const double* rp = nullptr;
+    if (idx < n) {
+      if (memory == BRILLE_MEM_HOST) rp = Q + 3 * idx;
+      else if (hipMemcpy(row, Q + 3 * idx, sizeof(row), hipMemcpyDeviceToHost) == hipSuccess) rp = row;
+    }
+    return report_point_error(e, rp);
+  }
+  return BRILLE_OK;
+}
+
+// ---- device plumbing -------------------------------------------------------
+int brille_device_alloc(uint64_t bytes, void** ptr) {
+  if (!ptr) return fail(BRILLE_ERR_INVALID_ARGUMENT, "null argument");
+  HIP_TRY(hipMalloc(ptr, bytes ? bytes : 1));
+  return BRILLE_OK;
+}
+int brille_device_free(void* ptr) {
+  HIP_TRY(hipFree(ptr));
+  return BRILLE_OK;
+}
+int brille_memcpy_h2d(void* dst, const void* src, uint64_t bytes) {
+  HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  return BRILLE_OK;
+}
+int brille_memcpy_d2h(void* dst, const void* src, uint64_t bytes) {
+  HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return BRILLE_OK;
+}
+int brille_stream_create(void** stream) {
+  hipStream_t s;
+  HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  *stream = s;
+  return BRILLE_OK;
+}
+int brille_stream_destroy(void* stream) {
+  HIP_TRY(hipStreamDestroy(static_cast<hipStream_t>(stream)));
+  return BRILLE_OK;
+}
+int brille_stream_sync(void* stream) {
+  HIP_TRY(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
+  return BRILLE_OK;
+}
+int brille_device_sync(void) {
+  HIP_TRY(hipDeviceSynchronize());
+  return BRILLE_OK;
+}
+
+}  // extern "C"

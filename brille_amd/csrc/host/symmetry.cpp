@@ -1,0 +1,476 @@
+// Space-group generation from Hall symbols, point groups and atom bases.
+#include "symmetry.hpp"
+
+#include <algorithm>
+#include <cctype>
+#include <cfloat>
+#include <map>
+#include <stdexcept>
+
+namespace brille_amd {
+
+namespace {
+
+double mod1(double x) {
+  double r = x - std::floor(x);
+  if (r > 1.0 - 1e-10 || r < 1e-10) r = 0.0;
+  return r;
+}
+V3 mod1(const V3& v) { return {mod1(v[0]), mod1(v[1]), mod1(v[2])}; }
+
+bool same_mod1(const V3& a, const V3& b, double tol) {
+  for (int i = 0; i < 3; ++i) {
+    double d = a[i] - b[i];
+    d -= std::round(d);
+    if (std::abs(d) > tol) return false;
+  }
+  return true;
+}
+
+}  // namespace
+
+Motion Motion::inverse() const {
+  Motion o;
+  o.W = inverse_int(W);
+  V3 t = mat_vec(to_double(o.W), w);
+  for (int i = 0; i < 3; ++i) t[i] = -t[i];
+  o.w = mod1(t);
+  return o;
+}
+
+Motion Motion::operator*(const Motion& o) const {
+  Motion r;
+  r.W = mat_mul(W, o.W);
+  V3 t = mat_vec(to_double(W), o.w);
+  for (int i = 0; i < 3; ++i) t[i] += w[i];
+  r.w = mod1(t);
+  return r;
+}
+
+bool Motion::equals(const Motion& o, double tol) const { return W == o.W && same_mod1(w, o.w, tol); }
+
+Symmetry Symmetry::from_matrices(const std::vector<IM3>& W, const std::vector<V3>& w) {
+  if (W.size() != w.size()) throw std::runtime_error("Symmetry requires one translation per matrix");
+  std::vector<Motion> m;
+  m.reserve(W.size());
+  for (size_t i = 0; i < W.size(); ++i) {
+    Motion x;
+    x.W = W[i];
+    x.w = mod1(w[i]);
+    m.push_back(x);
+  }
+  return Symmetry(std::move(m));
+}
+
+Symmetry Symmetry::generate() const {
+  std::vector<Motion> all;
+  auto contains = [&](const Motion& m) {
+    for (const auto& x : all)
+      if (x.equals(m)) return true;
+    return false;
+  };
+  all.push_back(Motion{});
+  for (const auto& g : ops)
+    if (!contains(g)) all.push_back(g);
+  size_t done = 0;
+  while (done < all.size()) {
+    size_t n = all.size();
+    for (size_t i = 0; i < n; ++i)
+      for (size_t j = (i < done ? done : 0); j < n; ++j) {
+        for (int order = 0; order < 2; ++order) {
+          Motion p = order ? all[j] * all[i] : all[i] * all[j];
+          if (!contains(p)) {
+            all.push_back(p);
+            if (all.size() > 768) throw std::runtime_error("symmetry generators do not close into a space group");
+          }
+        }
+      }
+    done = n;
+  }
+  return Symmetry(std::move(all));
+}
+
+Bravais Symmetry::centring() const {
+  bool a = false, b = false, c = false, i = false, r = false;
+  for (const auto& m : ops) {
+    if (!is_identity(m.W)) continue;
+    const V3& t = m.w;
+    auto is = [&](double x, double y, double z) { return same_mod1(t, V3{x, y, z}, 1e-8); };
+    if (is(0, 0.5, 0.5)) a = true;
+    if (is(0.5, 0, 0.5)) b = true;
+    if (is(0.5, 0.5, 0)) c = true;
+    if (is(0.5, 0.5, 0.5)) i = true;
+    if (is(2. / 3, 1. / 3, 1. / 3) || is(1. / 3, 2. / 3, 2. / 3)) r = true;
+  }
+  if (a && b && c) return Bravais::F;
+  if (r) return Bravais::R;
+  if (i) return Bravais::I;
+  if (a) return Bravais::A;
+  if (b) return Bravais::B;
+  if (c) return Bravais::C;
+  return Bravais::P;
+}
+
+size_t Symmetry::find_matrix_index(const IM3& R) const {
+  for (size_t i = 0; i < ops.size(); ++i)
+    if (ops[i].W == R) return i;
+  return ops.size();
+}
+
+bool Symmetry::has_space_inversion() const {
+  for (const auto& m : ops)
+    if (m.W == IM3{-1, 0, 0, 0, -1, 0, 0, 0, -1}) return true;
+  return false;
+}
+
+Symmetry Symmetry::add_space_inversion() const {
+  if (has_space_inversion()) return *this;
+  Symmetry g(*this);
+  Motion inv;
+  inv.W = IM3{-1, 0, 0, 0, -1, 0, 0, 0, -1};
+  g.ops.push_back(inv);
+  return g.generate();
+}
+
+// ---------------------------------------------------------------------------
+// Hall symbols
+// ---------------------------------------------------------------------------
+namespace {
+
+IM3 axis_matrix(int n, char axis, char principal) {
+  const IM3 E{1, 0, 0, 0, 1, 0, 0, 0, 1};
+  if (n == 1) return E;
+  if (axis == '\'' || axis == '"') {
+    // face-diagonal two-fold axes, relative to the preceding (principal) axis
+    bool dbl = axis == '"';
+    switch (principal) {
+      case 'x': return dbl ? IM3{-1, 0, 0, 0, 0, 1, 0, 1, 0} : IM3{-1, 0, 0, 0, 0, -1, 0, -1, 0};
+      case 'y': return dbl ? IM3{0, 0, 1, 0, -1, 0, 1, 0, 0} : IM3{0, 0, -1, 0, -1, 0, -1, 0, 0};
+      default: return dbl ? IM3{0, 1, 0, 1, 0, 0, 0, 0, -1} : IM3{0, -1, 0, -1, 0, 0, 0, 0, -1};
+    }
+  }
+  if (axis == '*') {
+    if (n != 3) throw std::runtime_error("the body-diagonal axis only carries three-fold rotations");
+    return IM3{0, 0, 1, 1, 0, 0, 0, 1, 0};
+  }
+  switch (axis) {
+    case 'x':
+      switch (n) {
+        case 2: return {1, 0, 0, 0, -1, 0, 0, 0, -1};
+        case 3: return {1, 0, 0, 0, 0, -1, 0, 1, -1};
+        case 4: return {1, 0, 0, 0, 0, -1, 0, 1, 0};
+        case 6: return {1, 0, 0, 0, 1, -1, 0, 1, 0};
+      }
+      break;
+    case 'y':
+      switch (n) {
+        case 2: return {-1, 0, 0, 0, 1, 0, 0, 0, -1};
+        case 3: return {-1, 0, 1, 0, 1, 0, -1, 0, 0};
+        case 4: return {0, 0, 1, 0, 1, 0, -1, 0, 0};
+        case 6: return {0, 0, 1, 0, 1, 0, -1, 0, 1};
+      }
+      break;
+    default:  // z
+      switch (n) {
+        case 2: return {-1, 0, 0, 0, -1, 0, 0, 0, 1};
+        case 3: return {0, -1, 0, 1, -1, 0, 0, 0, 1};
+        case 4: return {0, -1, 0, 1, 0, 0, 0, 0, 1};
+        case 6: return {1, -1, 0, 1, 0, 0, 0, 0, 1};
+      }
+  }
+  throw std::runtime_error("invalid Hall rotation order");
+}
+
+std::vector<std::string> split_ws(const std::string& s) {
+  std::vector<std::string> out;
+  std::string cur;
+  for (char c : s) {
+    if (std::isspace(static_cast<unsigned char>(c))) {
+      if (!cur.empty()) out.push_back(cur);
+      cur.clear();
+    } else {
+      cur.push_back(c);
+    }
+  }
+  if (!cur.empty()) out.push_back(cur);
+  return out;
+}
+
+}  // namespace
+
+Symmetry hall_generators(const std::string& hall_in, Bravais* bravais) {
+  std::string hall = hall_in;
+  // optional change-of-basis "(vx vy vz)" in twelfths of the cell edges
+  V3 shift{0, 0, 0};
+  bool has_shift = false;
+  auto lp = hall.find('(');
+  if (lp != std::string::npos) {
+    auto rp = hall.find(')', lp);
+    if (rp == std::string::npos) throw std::runtime_error("unbalanced Hall change-of-basis");
+    auto parts = split_ws(hall.substr(lp + 1, rp - lp - 1));
+    if (parts.size() != 3) throw std::runtime_error("Hall change-of-basis needs three numbers");
+    for (int i = 0; i < 3; ++i) shift[i] = std::stod(parts[i]) / 12.0;
+    has_shift = true;
+    hall = hall.substr(0, lp);
+  }
+  auto tokens = split_ws(hall);
+  if (tokens.empty()) throw std::runtime_error("empty Hall symbol");
+  std::string L = tokens[0];
+  bool centro = false;
+  if (!L.empty() && L[0] == '-') {
+    centro = true;
+    L = L.substr(1);
+  }
+  if (L.size() != 1) throw std::runtime_error("invalid Hall lattice symbol '" + tokens[0] + "'");
+  std::vector<Motion> gens;
+  Bravais bv = Bravais::P;
+  auto add_t = [&](double x, double y, double z) {
+    Motion m;
+    m.w = {x, y, z};
+    gens.push_back(m);
+  };
+  switch (std::toupper(static_cast<unsigned char>(L[0]))) {
+    case 'P': bv = Bravais::P; break;
+    case 'A': bv = Bravais::A; add_t(0, 0.5, 0.5); break;
+    case 'B': bv = Bravais::B; add_t(0.5, 0, 0.5); break;
+    case 'C': bv = Bravais::C; add_t(0.5, 0.5, 0); break;
+    case 'I': bv = Bravais::I; add_t(0.5, 0.5, 0.5); break;
+    case 'R': bv = Bravais::R; add_t(2. / 3, 1. / 3, 1. / 3); add_t(1. / 3, 2. / 3, 2. / 3); break;
+    case 'F': bv = Bravais::F; add_t(0, 0.5, 0.5); add_t(0.5, 0, 0.5); add_t(0.5, 0.5, 0); break;
+    default: throw std::runtime_error("unsupported Hall lattice symbol '" + L + "'");
+  }
+  if (centro) {
+    Motion inv;
+    inv.W = IM3{-1, 0, 0, 0, -1, 0, 0, 0, -1};
+    gens.push_back(inv);
+  }
+  int prev_n = 0;
+  char prev_axis = 'z';
+  for (size_t ti = 1; ti < tokens.size(); ++ti) {
+    const std::string& tok = tokens[ti];
+    size_t p = 0;
+    bool improper = false;
+    if (tok[p] == '-') {
+      improper = true;
+      ++p;
+    }
+    if (p >= tok.size() || !std::isdigit(static_cast<unsigned char>(tok[p])))
+      throw std::runtime_error("invalid Hall matrix symbol '" + tok + "'");
+    int n = tok[p++] - '0';
+    if (n != 1 && n != 2 && n != 3 && n != 4 && n != 6) throw std::runtime_error("invalid Hall rotation '" + tok + "'");
+    char axis = 0;
+    int screw = 0;
+    V3 t{0, 0, 0};
+    for (; p < tok.size(); ++p) {
+      char c = tok[p];
+      switch (c) {
+        case 'x': case 'y': case 'z': case '\'': case '"': case '*': axis = c; break;
+        case '1': case '2': case '3': case '4': case '5': screw = c - '0'; break;
+        case 'a': t[0] += 0.5; break;
+        case 'b': t[1] += 0.5; break;
+        case 'c': t[2] += 0.5; break;
+        case 'n': t[0] += 0.5; t[1] += 0.5; t[2] += 0.5; break;
+        case 'u': t[0] += 0.25; break;
+        case 'v': t[1] += 0.25; break;
+        case 'w': t[2] += 0.25; break;
+        case 'd': t[0] += 0.25; t[1] += 0.25; t[2] += 0.25; break;
+        default: throw std::runtime_error("invalid character in Hall matrix symbol '" + tok + "'");
+      }
+    }
+    const size_t index = ti - 1;
+    if (!axis) {
+      if (index == 0) axis = 'z';
+      else if (index == 1 && n == 2) axis = (prev_n == 2 || prev_n == 4) ? 'x' : '\'';
+      else if (index == 2 && n == 3) axis = '*';
+      else if (n == 1) axis = 'z';
+      else throw std::runtime_error("cannot infer the axis of Hall matrix symbol '" + tok + "'");
+    }
+    char principal = prev_axis;
+    IM3 R = axis_matrix(n, axis, principal);
+    if (screw) {
+      double f = static_cast<double>(screw) / static_cast<double>(n);
+      if (axis == 'x') t[0] += f;
+      else if (axis == 'y') t[1] += f;
+      else t[2] += f;
+    }
+    if (improper)
+      for (auto& x : R) x = -x;
+    Motion m;
+    m.W = R;
+    m.w = t;
+    gens.push_back(m);
+    prev_n = n;
+    if (axis == 'x' || axis == 'y' || axis == 'z') prev_axis = axis;
+  }
+  if (has_shift) {
+    // W' = V W V^-1 with V = (E, v): t' = t + v - W v
+    for (auto& m : gens) {
+      V3 Wv = mat_vec(to_double(m.W), shift);
+      for (int i = 0; i < 3; ++i) m.w[i] = m.w[i] + shift[i] - Wv[i];
+    }
+  }
+  for (auto& m : gens) m.w = mod1(m.w);
+  if (bravais) *bravais = bv;
+  return Symmetry(std::move(gens));
+}
+
+std::string hall_symbol_of(int n) {
+  // Default ITA settings (unique axis b, first cell/origin choice, hexagonal
+  // axes for R groups), one Hall symbol per space-group number.
+  static const char* table[231] = {
+      "",
+      "P 1", "-P 1", "P 2y", "P 2yb", "C 2y", "P -2y", "P -2yc", "C -2y", "C -2yc", "-P 2y",
+      "-P 2yb", "-C 2y", "-P 2yc", "-P 2ybc", "-C 2yc", "P 2 2", "P 2c 2", "P 2 2ab", "P 2ac 2ab", "C 2c 2",
+      "C 2 2", "F 2 2", "I 2 2", "I 2b 2c", "P 2 -2", "P 2c -2", "P 2 -2c", "P 2 -2a", "P 2c -2ac", "P 2 -2bc",
+      "P 2ac -2", "P 2 -2ab", "P 2c -2n", "P 2 -2n", "C 2 -2", "C 2c -2", "C 2 -2c", "A 2 -2", "A 2 -2c", "A 2 -2a",
+      "A 2 -2ac", "F 2 -2", "F 2 -2d", "I 2 -2", "I 2 -2c", "I 2 -2a", "-P 2 2", "P 2 2 -1n", "-P 2 2c", "P 2 2 -1ab",
+      "-P 2a 2a", "-P 2a 2bc", "-P 2ac 2", "-P 2a 2ac", "-P 2 2ab", "-P 2ab 2ac", "-P 2c 2b", "-P 2 2n", "P 2 2ab -1ab", "-P 2n 2ab",
+      "-P 2ac 2ab", "-P 2ac 2n", "-C 2c 2", "-C 2ac 2", "-C 2 2", "-C 2 2c", "-C 2a 2", "C 2 2 -1ac", "-F 2 2", "F 2 2 -1d",
+      "-I 2 2", "-I 2 2c", "-I 2b 2c", "-I 2b 2", "P 4", "P 4w", "P 4c", "P 4cw", "I 4", "I 4bw",
+      "P -4", "I -4", "-P 4", "-P 4c", "P 4ab -1ab", "P 4n -1n", "-I 4", "I 4bw -1bw", "P 4 2", "P 4ab 2ab",
+      "P 4w 2c", "P 4abw 2nw", "P 4c 2", "P 4n 2n", "P 4cw 2c", "P 4nw 2abw", "I 4 2", "I 4bw 2bw", "P 4 -2", "P 4 -2ab",
+      "P 4c -2c", "P 4n -2n", "P 4 -2c", "P 4 -2n", "P 4c -2", "P 4c -2ab", "I 4 -2", "I 4 -2c", "I 4bw -2", "I 4bw -2c",
+      "P -4 2", "P -4 2c", "P -4 2ab", "P -4 2n", "P -4 -2", "P -4 -2c", "P -4 -2ab", "P -4 -2n", "I -4 -2", "I -4 -2c",
+      "I -4 2", "I -4 2bw", "-P 4 2", "-P 4 2c", "P 4 2 -1ab", "P 4 2 -1n", "-P 4 2ab", "-P 4 2n", "P 4ab 2ab -1ab", "P 4ab 2n -1ab",
+      "-P 4c 2", "-P 4c 2c", "P 4n 2c -1n", "P 4n 2 -1n", "-P 4c 2ab", "-P 4n 2n", "P 4n 2n -1n", "P 4n 2ab -1n", "-I 4 2", "-I 4 2c",
+      "I 4bw 2bw -1bw", "I 4bw 2aw -1bw", "P 3", "P 31", "P 32", "R 3", "-P 3", "-R 3", "P 3 2", "P 3 2\"",
+      "P 31 2c (0 0 1)", "P 31 2\"", "P 32 2c (0 0 -1)", "P 32 2\"", "R 3 2\"", "P 3 -2\"", "P 3 -2", "P 3 -2\"c", "P 3 -2c", "R 3 -2\"",
+      "R 3 -2\"c", "-P 3 2", "-P 3 2c", "-P 3 2\"", "-P 3 2\"c", "-R 3 2\"", "-R 3 2\"c", "P 6", "P 61", "P 65",
+      "P 62", "P 64", "P 6c", "P -6", "-P 6", "-P 6c", "P 6 2", "P 61 2 (0 0 -1)", "P 65 2 (0 0 1)", "P 62 2c (0 0 1)",
+      "P 64 2c (0 0 -1)", "P 6c 2c", "P 6 -2", "P 6 -2c", "P 6c -2", "P 6c -2c", "P -6 2", "P -6c 2", "P -6 -2", "P -6c -2c",
+      "-P 6 2", "-P 6 2c", "-P 6c 2", "-P 6c 2c", "P 2 2 3", "F 2 2 3", "I 2 2 3", "P 2ac 2ab 3", "I 2b 2c 3", "-P 2 2 3",
+      "P 2 2 3 -1n", "-F 2 2 3", "F 2 2 3 -1d", "-I 2 2 3", "-P 2ac 2ab 3", "-I 2b 2c 3", "P 4 2 3", "P 4n 2 3", "F 4 2 3", "F 4d 2 3",
+      "I 4 2 3", "P 4acd 2ab 3", "P 4bd 2ab 3", "I 4bd 2c 3", "P -4 2 3", "F -4 2 3", "I -4 2 3", "P -4n 2 3", "F -4c 2 3", "I -4bd 2c 3",
+      "-P 4 2 3", "P 4 2 3 -1n", "-P 4n 2 3", "P 4n 2 3 -1n", "-F 4 2 3", "-F 4c 2 3", "F 4d 2 3 -1d", "F 4d 2 3 -1cd", "-I 4 2 3", "-I 4bd 2c 3"};
+  if (n < 1 || n > 230) return "";
+  return table[n];
+}
+
+Symmetry spacegroup_from_string(const std::string& s_in, Bravais* bravais) {
+  std::string s = s_in;
+  // trim
+  while (!s.empty() && std::isspace(static_cast<unsigned char>(s.front()))) s.erase(s.begin());
+  while (!s.empty() && std::isspace(static_cast<unsigned char>(s.back()))) s.pop_back();
+  bool all_digits = !s.empty() && std::all_of(s.begin(), s.end(), [](char c) { return std::isdigit(static_cast<unsigned char>(c)); });
+  std::string hall;
+  if (all_digits) {
+    hall = hall_symbol_of(std::stoi(s));
+    if (hall.empty()) throw std::runtime_error("unknown space group number " + s);
+  } else {
+    std::string key;
+    for (char c : s)
+      if (!std::isspace(static_cast<unsigned char>(c)) && c != '_') key.push_back(c);
+    static const std::map<std::string, int> hm{
+        {"P1", 1}, {"P-1", 2}, {"P2/m", 10}, {"C2/m", 12}, {"P21/c", 14}, {"C2/c", 15}, {"P222", 16},
+        {"Pmmm", 47}, {"Pnma", 62}, {"Cmcm", 63}, {"Cmce", 64}, {"Cmca", 64}, {"Fmmm", 69}, {"Fddd", 70},
+        {"Immm", 71}, {"P4/mmm", 123}, {"P4/nmm", 129}, {"I4/mmm", 139}, {"I41/amd", 141}, {"P-3m1", 164},
+        {"R-3m", 166}, {"R-3c", 167}, {"P6/mmm", 191}, {"P63/mmc", 194}, {"P3221", 154}, {"P3121", 152},
+        {"Pm-3m", 221}, {"Fm-3m", 225}, {"Fd-3m", 227}, {"Im-3m", 229}, {"Ia-3d", 230}, {"F-43m", 216},
+        {"P6_3/mmc", 194}, {"P2_1/c", 14}};
+    auto it = hm.find(key);
+    if (it != hm.end()) hall = hall_symbol_of(it->second);
+    else hall = s;  // treat as a Hall symbol
+  }
+  Symmetry g = hall_generators(hall, bravais).generate();
+  return g;
+}
+
+// ---------------------------------------------------------------------------
+// point groups
+// ---------------------------------------------------------------------------
+int rotation_order(const IM3& R) {
+  IM3 M = R;
+  for (int n = 1; n <= 6; ++n) {
+    if (is_identity(M)) return n;
+    M = mat_mul(M, R);
+  }
+  throw std::runtime_error("matrix is not a crystallographic rotation");
+}
+
+int isometry_value(const IM3& R) {
+  int o = rotation_order(R);
+  return det3(R) < 0 ? -o : o;
+}
+
+PointSymmetry PointSymmetry::from_spacegroup(const Symmetry& s) {
+  std::vector<IM3> r;
+  for (const auto& m : s.ops)
+    if (std::find(r.begin(), r.end(), m.W) == r.end()) r.push_back(m.W);
+  PointSymmetry p(std::move(r));
+  p.sort_by_order();
+  return p;
+}
+
+void PointSymmetry::sort_by_order() {
+  std::stable_sort(R.begin(), R.end(), [](const IM3& a, const IM3& b) { return rotation_order(a) < rotation_order(b); });
+}
+
+size_t PointSymmetry::find_index(const IM3& a) const {
+  for (size_t i = 0; i < R.size(); ++i)
+    if (R[i] == a) return i;
+  return R.size();
+}
+
+size_t PointSymmetry::find_identity_index() const { return find_index(IM3{1, 0, 0, 0, 1, 0, 0, 0, 1}); }
+
+size_t PointSymmetry::get_inverse_index(size_t i) const {
+  if (i >= R.size()) return i;
+  for (size_t j = 0; j < R.size(); ++j)
+    if (is_identity(mat_mul(R[j], R[i]))) return j;
+  throw std::runtime_error("Incomplete pointgroup. Missing inverse of operation.");
+}
+
+std::vector<uint32_t> PointSymmetry::inverse_table() const {
+  std::vector<uint32_t> t(R.size());
+  for (size_t i = 0; i < R.size(); ++i) t[i] = static_cast<uint32_t>(get_inverse_index(i));
+  return t;
+}
+
+bool PointSymmetry::has_space_inversion() const {
+  for (const auto& r : R)
+    if (isometry_value(r) == -1) return true;
+  return false;
+}
+
+PointSymmetry PointSymmetry::add_space_inversion() const {
+  if (has_space_inversion()) return *this;
+  std::vector<IM3> r = R;
+  const size_t n = R.size();
+  for (size_t i = 0; i < n; ++i) {
+    IM3 m = R[i];
+    for (auto& x : m) x = -x;
+    r.push_back(m);
+  }
+  PointSymmetry p(std::move(r));
+  p.sort_by_order();
+  return p;
+}
+
+// ---------------------------------------------------------------------------
+// basis
+// ---------------------------------------------------------------------------
+bool Basis::equivalent_after_operation(size_t k, const Motion& op, double e_tol, int n_tol, uint32_t* l) const {
+  if (k >= positions.size()) throw std::runtime_error("invalid atom position index");
+  V3 K = op.move_point(positions[k]);
+  double Trel = DBL_EPSILON * (static_cast<double>(n_tol) * 10000.0);
+  double Tabs = 5.0 / 1000000000000000.0;
+  if (e_tol > Trel) Trel = e_tol;
+  if (e_tol > Tabs) Tabs = e_tol;
+  for (size_t i = 0; i < positions.size(); ++i) {
+    if (types[i] != types[k]) continue;
+    bool ok = true;
+    for (int j = 0; j < 3 && ok; ++j) {
+      double d = K[j] - positions[i][j] + 0.5;
+      d = std::abs(d - std::floor(d)) - 0.5;
+      double x = std::abs(d);
+      ok = x <= Tabs + Trel * std::abs(d) || x < DBL_MIN;
+    }
+    if (ok) {
+      *l = static_cast<uint32_t>(i);
+      return true;
+    }
+  }
+  return false;
+}
+
+}  // namespace brille_amd

@@ -1,0 +1,273 @@
+// Launch logic behind brille_ir_interpolate_at: validation, staging of host
+// buffers, kernel selection, error reduction and event timing.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "grid_state.hpp"
+
+using namespace brille_amd;
+
+namespace brille_amd {
+
+#define HIP_TRY(expr)                                                                                            \
+  do {                                                                                                           \
+    hipError_t e_ = (expr);                                                                                      \
+    if (e_ != hipSuccess) return fail(BRILLE_ERR_DEVICE, std::string(#expr ": ") + hipGetErrorString(e_));       \
+  } while (0)
+
+__global__ void k_err_init2(ErrRec* e) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    e->first_bad = ~0ull;
+    e->count = 0ull;
+  }
+}
+
+void free_staging(brille_grid* G) {
+  if (G->stage) (void)hipFree(G->stage);
+  G->stage = nullptr;
+  G->stage_bytes = 0;
+}
+
+// Interpolator::rotate_in_place dispatch  src/interpolator.hpp:387-428
+int check_rotation_modes(const brille_grid* G) {
+  const DataConst& d = G->d;
+  auto one = [&](int rot, int lu, bool cplx, bool has_vm, const char* what) -> int {
+    switch (lu) {
+      case BRILLE_LU_REAL_LATTICE:
+        if (rot == BRILLE_ROT_VECTOR || rot == BRILLE_ROT_PSEUDOVECTOR) return BRILLE_OK;
+        if (rot == BRILLE_ROT_GAMMA) {
+          if (!cplx) return fail(BRILLE_ERR_ROTATION, "RotatesLike == Gamma requires complex valued data!");
+          return BRILLE_OK;
+        }
+        return fail(BRILLE_ERR_ROTATION, "Impossible RotatesLike value!");
+      case BRILLE_LU_RECIPROCAL_LATTICE:
+        if (rot == BRILLE_ROT_VECTOR) return BRILLE_OK;
+        break;
+      case BRILLE_LU_ANGSTROM:
+        if (rot == BRILLE_ROT_GAMMA) {
+          if (!cplx) return fail(BRILLE_ERR_ROTATION, "RotatesLike == Gamma requires complex valued data!");
+          return BRILLE_OK;
+        }
+        break;
+      default:
+        break;
+    }
+    (void)has_vm;
+    return fail(BRILLE_ERR_ROTATION, std::string("LengthUnit, RotatesLike combination not implemented (") + what + ")");
+  };
+  int rc = one(d.val_rot, d.val_lu, false, d.ev[1] + d.ev[2] > 0, "values");
+  if (rc != BRILLE_OK) return rc;
+  rc = one(d.vec_rot, d.vec_lu, true, d.ec[1] + d.ec[2] > 0, "vectors");
+  if (rc != BRILLE_OK) return rc;
+  if (d.vec_rot == BRILLE_ROT_GAMMA && d.ec[1] + d.ec[2] > 0) {
+    if (!d.F0) return fail(BRILLE_ERR_ROTATION, "Gamma rotation requires a GammaTable");
+    if (d.ec[1] / 3u > d.n_atoms) return fail(BRILLE_ERR_ROTATION, "more eigenvector atoms than GammaTable atoms");
+    if (d.ec[2] > 0) return fail(BRILLE_ERR_UNSUPPORTED, "Gamma transformation of 3N x 3N tensors is not supported");
+    if (d.vec_lu == BRILLE_LU_ANGSTROM && !d.cart_rot) return fail(BRILLE_ERR_ROTATION, "Cartesian Gamma rotation needs Cartesian matrices");
+  }
+  return BRILLE_OK;
+}
+
+template <typename Real, bool Fast>
+static void launch(const brille_grid* G, const LaunchArgs& a, hipStream_t s) {
+  const uint64_t blocks = (a.n + TILE - 1) / TILE;
+  if (!blocks) return;
+  hipLaunchKernelGGL((k_ir_interpolate<Real, Fast>), dim3((unsigned)blocks), dim3(TILE), 0, s, G->g, G->d, a);
+}
+
+static void launch_sel(const brille_grid* G, const LaunchArgs& a, int precision, bool fast, hipStream_t s) {
+  if (precision == BRILLE_FP32) {
+    if (fast) launch<float, true>(G, a, s);
+    else launch<float, false>(G, a, s);
+  } else {
+    if (fast) launch<double, true>(G, a, s);
+    else launch<double, false>(G, a, s);
+  }
+}
+
+static int report(const brille_grid* G, const ErrRec& e, const double* Q, uint64_t n, bool host_mem) {
+  const uint64_t idx = e.first_bad >> 3;
+  const int st = (int)(e.first_bad & 7ull);
+  double row[3];
+  const double* rp = nullptr;
+  if (idx < n) {
+    if (host_mem) rp = Q + 3 * idx;
+    else if (hipMemcpy(row, Q + 3 * idx, sizeof(row), hipMemcpyDeviceToHost) == hipSuccess) rp = row;
+  }
+  char buf[200];
+  std::string q = "#" + std::to_string(idx);
+  if (rp) {
+    std::snprintf(buf, sizeof(buf), "[%.17g, %.17g, %.17g]", rp[0], rp[1], rp[2]);
+    q = buf;
+  }
+  (void)G;
+  switch (st) {
+    case BRILLE_PT_OUTSIDE_BZ:
+      return fail(BRILLE_ERR_OUTSIDE_BZ, "Not all points inside Brillouin zone (first: Q = " + q + ")");
+    case BRILLE_PT_OUTSIDE_IR:
+      return fail(BRILLE_ERR_OUTSIDE_IR, "Q = " + q + " is outside of the irreducible BrillouinZone");
+    case BRILLE_PT_NULL_NODE:
+      return fail(BRILLE_ERR_NULL_NODE, "Attempting to access null node for point " + q);
+    default: {
+      std::string m = "interpolate_at failed to find " + std::to_string(e.count) + " point";
+      m += e.count > 1 ? "s." : ".";
+      return fail(BRILLE_ERR_NOT_FOUND, m);
+    }
+  }
+}
+
+int run_interpolate(brille_grid* G, const double* Q, uint64_t n, const brille_run_opts* opts_in, void* values_out,
+                    void* vectors_out, const brille_aux_out* aux_in, hipStream_t s, brille_kernel_times* times,
+                    uint32_t iters) {
+  if (!G) return fail(BRILLE_ERR_INVALID_ARGUMENT, "null grid");
+  if (!G->filled) return fail(BRILLE_ERR_NOT_FILLED, "The interpolation data must be filled before interpolating.");
+  if (G->g.n_nodes == 0) return fail(BRILLE_ERR_INVALID_ARGUMENT, "this handle holds no trellis");
+  brille_run_opts opts{};
+  opts.check = 1;
+  if (opts_in) opts = *opts_in;
+  if (opts.precision != BRILLE_FP64 && opts.precision != BRILLE_FP32)
+    return fail(BRILLE_ERR_INVALID_ARGUMENT, "unknown precision");
+  int rc = check_rotation_modes(G);
+  if (rc != BRILLE_OK) return rc;
+  if (n == 0) return BRILLE_OK;
+  if (!Q || !values_out || !vectors_out) return fail(BRILLE_ERR_INVALID_ARGUMENT, "null input or output pointer");
+  if (opts.use_proposal && !G->prop_trained)
+    return fail(BRILLE_ERR_INVALID_ARGUMENT, "use_proposal requires a trained proposal (brille_proposal_train)");
+  const bool fast = opts.kernel != BRILLE_KERNEL_PLAIN;
+  const size_t rs = opts.precision == BRILLE_FP32 ? 4 : 8;
+  const size_t vbytes = (size_t)G->d.B * G->d.sv * rs;
+  const size_t cbytes = (size_t)G->d.B * G->d.sc * 2 * rs;
+  brille_aux_out aux{};
+  if (aux_in) aux = *aux_in;
+  LaunchArgs a{};
+  a.no_move = opts.no_move;
+  a.use_proposal = opts.use_proposal;
+  a.err = G->err;
+  if (opts.memory == BRILLE_MEM_DEVICE) {
+    a.Q = Q;
+    a.n = n;
+    a.base = 0;
+    a.vals = values_out;
+    a.vecs = vectors_out;
+    a.aux = aux;
+    hipLaunchKernelGGL(k_err_init2, dim3(1), dim3(1), 0, s, G->err);
+    if (times) {
+      hipEvent_t e0, e1;
+      HIP_TRY(hipEventCreate(&e0));
+      HIP_TRY(hipEventCreate(&e1));
+      HIP_TRY(hipEventRecord(e0, s));
+      for (uint32_t it = 0; it < std::max(iters, 1u); ++it) launch_sel(G, a, opts.precision, fast, s);
+      HIP_TRY(hipEventRecord(e1, s));
+      HIP_TRY(hipGetLastError());
+      HIP_TRY(hipEventSynchronize(e1));
+      float ms = 0;
+      HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      std::memset(times, 0, sizeof(*times));
+      times->n_kernels = 1;
+      std::snprintf(times->names[0], sizeof(times->names[0]), "k_ir_interpolate");
+      times->ms[0] = ms / (float)std::max(iters, 1u);
+      times->total_ms = times->ms[0];
+    } else {
+      launch_sel(G, a, opts.precision, fast, s);
+    }
+    HIP_TRY(hipGetLastError());
+    if (!opts.check && !times) return BRILLE_OK;
+    ErrRec e;
+    HIP_TRY(hipMemcpyAsync(&e, G->err, sizeof(e), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (e.count) return report(G, e, Q, n, false);
+    return BRILLE_OK;
+  }
+  // host memory: stream through device staging in chunks
+  const uint64_t chunk = std::min<uint64_t>(n, 1ull << 20);
+  struct Slot { void* host; size_t per; size_t off; };
+  std::vector<Slot> slots;
+  size_t off = 0;
+  auto add = [&](void* h, size_t per) {
+    slots.push_back({h, per, off});
+    off += (per * chunk + 255) & ~size_t(255);
+  };
+  add(const_cast<double*>(Q), 3 * sizeof(double));
+  add(values_out, vbytes);
+  add(vectors_out, cbytes);
+  void** auxp[10] = {(void**)&aux.ir_q, (void**)&aux.tau, (void**)&aux.rot, (void**)&aux.invrot, (void**)&aux.node,
+                     (void**)&aux.tet, (void**)&aux.n_iw, (void**)&aux.iw_index, (void**)&aux.iw_weight,
+                     (void**)&aux.status};
+  const size_t auxper[10] = {24, 12, 4, 4, 4, 4, 1, 32, 64, 1};
+  for (int i = 0; i < 10; ++i)
+    if (*auxp[i]) add(*auxp[i], auxper[i]);
+  if (G->stage_bytes < off) {
+    free_staging(G);
+    HIP_TRY(hipMalloc(&G->stage, off));
+    G->stage_bytes = off;
+  }
+  char* st = static_cast<char*>(G->stage);
+  hipLaunchKernelGGL(k_err_init2, dim3(1), dim3(1), 0, s, G->err);
+  for (uint64_t c0 = 0; c0 < n; c0 += chunk) {
+    const uint64_t m = std::min(chunk, n - c0);
+    HIP_TRY(hipMemcpyAsync(st + slots[0].off, Q + 3 * c0, m * 3 * sizeof(double), hipMemcpyHostToDevice, s));
+    a.Q = reinterpret_cast<const double*>(st + slots[0].off);
+    a.n = m;
+    a.base = c0;
+    a.vals = st + slots[1].off;
+    a.vecs = st + slots[2].off;
+    brille_aux_out da{};
+    size_t k = 3;
+    void** dauxp[10] = {(void**)&da.ir_q, (void**)&da.tau, (void**)&da.rot, (void**)&da.invrot, (void**)&da.node,
+                        (void**)&da.tet, (void**)&da.n_iw, (void**)&da.iw_index, (void**)&da.iw_weight,
+                        (void**)&da.status};
+    for (int i = 0; i < 10; ++i)
+      if (*auxp[i]) *dauxp[i] = st + slots[k++].off;
+    a.aux = da;
+    launch_sel(G, a, opts.precision, fast, s);
+    HIP_TRY(hipGetLastError());
+    for (size_t i = 1; i < slots.size(); ++i)
+      HIP_TRY(hipMemcpyAsync(static_cast<char*>(slots[i].host) + c0 * slots[i].per, st + slots[i].off, m * slots[i].per,
+                             hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+  }
+  ErrRec e;
+  HIP_TRY(hipMemcpyAsync(&e, G->err, sizeof(e), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  if (e.count) return report(G, e, Q, n, true);
+  return BRILLE_OK;
+}
+
+int build_fast_tables(const brille_grid_desc* desc, brille_grid* G) {
+  (void)desc;
+  (void)G;
+  return BRILLE_OK;
+}
+
+void free_proposal(brille_grid* G) {
+  for (void* p : G->prop_allocs) (void)hipFree(p);
+  G->prop_allocs.clear();
+  G->prop_trained = false;
+}
+
+}  // namespace brille_amd
+
+extern "C" {
+
+int brille_time_ir_interpolate_at(brille_grid* G, const double* Q, uint64_t n, const brille_run_opts* opts,
+                                  void* values_out, void* vectors_out, uint32_t iters, void* stream,
+                                  brille_kernel_times* times) {
+  if (!times) return fail(BRILLE_ERR_INVALID_ARGUMENT, "null times");
+  if (opts && opts->memory != BRILLE_MEM_DEVICE) return fail(BRILLE_ERR_INVALID_ARGUMENT, "timing needs device memory");
+  return run_interpolate(G, Q, n, opts, values_out, vectors_out, nullptr, static_cast<hipStream_t>(stream), times, iters);
+}
+
+int brille_proposal_stats(const brille_grid* G, uint64_t* tried, uint64_t* hits) {
+  if (!G) return fail(BRILLE_ERR_INVALID_ARGUMENT, "null grid");
+  if (tried) *tried = G->last_tried;
+  if (hits) *hits = G->last_hits;
+  return BRILLE_OK;
+}
+
+}  // extern "C"
