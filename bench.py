@@ -1,0 +1,163 @@
+#!/usr/bin/env python3
+"""Benchmark of BZTrellisQdc.ir_interpolate_at on one MI355X.
+
+Workload (default): the learned-proposal configuration -- hexagonal P6_3/mmc,
+a=b=3.2 A, c=5.2 A, 2 atoms, 6 modes with complex eigenvectors, trellis node
+volume 1e-5 of the irreducible volume, 1e8 Q uniform in [-4,4]^3 rlu,
+inputs and outputs resident in HBM.  One step = one ir_interpolate_at pass over
+all Q.  The headline path is the tuned FP32 kernel (FP64 fold-in, indices and
+weights; FP32/complex64 sums); the plain FP64 kernel is timed beside it.
+
+Prints one JSON line.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def table_bytes(t):
+    keys = ["prim_plane_a", "prim_plane_b", "prim_plane_c", "prim_normals", "prim_taus", "prim_tau_lens",
+            "outer_plane_a", "outer_plane_b", "outer_plane_c", "wedge_normals", "rotations", "inverse_index",
+            "node_type", "node_index", "cube_vertices", "poly_tet_offset", "tet_vertices", "tet_circumsphere",
+            "tet_volume", "vertices"]
+    return int(sum(np.asarray(t[k]).nbytes for k in keys) + sum(np.asarray(k).nbytes for k in t["knots"]))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", type=int, default=4)
+    ap.add_argument("--n-q", type=float, default=None)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-plain", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(the path shards by Q with no exchange; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from brille_amd import capi
+    from brille_amd import synthetic as S
+
+    lib = capi.lib()
+    if lib.brille_device_count() < 1:
+        raise SystemExit("no HIP device")
+    cfg = args.config
+    c = S.CONFIGS[cfg]
+    n = int(args.n_q if args.n_q is not None else c["n_q"])
+    t0 = time.time()
+    g = S.grid(cfg)
+    T = g.tables()
+    log(f"grid cfg{cfg}: {T['vertices'].shape[0]} vertices, {T['node_type'].size} nodes, "
+        f"{T['tet_volume'].size} tetrahedra, built in {time.time() - t0:.1f}s")
+    dev = capi.Grid(T)
+    B = int(T["branches"])
+    sv = int(sum(T["values_elements"]))
+    sc = int(sum(T["vectors_elements"]))
+    Q = S.q_points(cfg, n)
+    dQ = capi.DeviceArray.from_host(Q)
+    vs, cs = dev.out_shapes(n)
+    dv = capi.DeviceArray(vs, np.float64)
+    dc = capi.DeviceArray(cs, np.complex128)
+    stream = C.c_void_p()
+    capi.check(lib.brille_stream_create(C.byref(stream)))
+
+    def run(opts, iters):
+        times = capi.KernelTimes()
+        capi.check(lib.brille_time_ir_interpolate_at(dev.h, dQ.ptr, n, C.byref(opts), dv.ptr, dc.ptr, iters,
+                                                     stream, C.byref(times)))
+        return times
+
+    def timed(opts, label):
+        # one checked call first: any failing point aborts the benchmark
+        o = capi.RunOpts(*[getattr(opts, f) for f, _ in capi.RunOpts._fields_])
+        o.check = 1
+        capi.check(lib.brille_ir_interpolate_at(dev.h, dQ.ptr, n, C.byref(o), dv.ptr, dc.ptr, None, stream))
+        for _ in range(args.warmup):
+            run(opts, 1)
+        capi.check(lib.brille_device_sync())
+        kms = []
+        w0 = time.perf_counter()
+        for s in range(args.steps):
+            kms.append(run(opts, 1).ms[0])
+        capi.check(lib.brille_device_sync())
+        wall = (time.perf_counter() - w0) * 1000.0 / args.steps
+        log(f"{label}: {wall:.3f} ms/step wall, kernel {np.mean(kms):.3f} ms, {n / wall * 1e3:.4g} Q/s")
+        return wall, float(np.mean(kms))
+
+    fast = capi.RunOpts(0, capi.FP32, 0, capi.MEM_DEVICE, capi.KERNEL_TUNED, 0)
+    wall_fast, k_fast = timed(fast, "tuned fp32")
+    plain_res = None
+    if not args.no_plain:
+        plain = capi.RunOpts(0, capi.FP64, 0, capi.MEM_DEVICE, capi.KERNEL_PLAIN, 0)
+        wall_plain, k_plain = timed(plain, "plain fp64")
+        plain_res = dict(value=n / wall_plain * 1e3, ms_per_step=wall_plain, kernel_ms=k_plain,
+                         dtype="f64", bytes_per_q=24 + B * sv * 8 + B * sc * 16)
+    capi.check(lib.brille_stream_destroy(stream))
+
+    bytes_per_q = 24 + B * sv * 4 + B * sc * 8
+    tb = table_bytes(T)
+    alg = n * bytes_per_q + tb
+    achieved = alg / (k_fast * 1e-3) / 1e9
+    roofline = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
+                    traffic=None, bytes_per_q=bytes_per_q, table_bytes=tb, kernel="k_ir_interpolate<float,true>")
+
+    # CPU baseline: the oracle (C, OpenMP, the reference's loop structure) on a bounded sample
+    from oracle import oracle as O
+    cores = min(16, os.cpu_count() or 1)
+    probe = S.q_points(cfg, 20000, stream=7)
+    tp = time.perf_counter()
+    O.ir_interpolate_at(T, probe, nthreads=cores)
+    rate = len(probe) / (time.perf_counter() - tp)
+    m = int(min(max(rate * args.cpu_seconds, 20000), 5e6))
+    sample = S.q_points(cfg, m, stream=8)
+    tc = time.perf_counter()
+    rc, *_ = O.ir_interpolate_at(T, sample, nthreads=cores)
+    cpu_s = time.perf_counter() - tc
+    cpu = dict(value=m / cpu_s, unit="Q-points/s", cores=cores, kind="port",
+               sample=f"{m} Q of the same workload (uniform in [-{c['box']},{c['box']}]^3), FP64, {cores} OpenMP threads")
+    log(f"cpu baseline: {m / cpu_s:.4g} Q/s on {cores} threads")
+
+    out = {
+        "metric": "Q-points/s ir_interpolate_at, 1 MI355X: learned+batched vs plain, %HBM peak",
+        "value": n / wall_fast * 1e3,
+        "unit": "Q-points/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall_fast,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"config{cfg}: {c['name']}, node volume {c['volume_fraction']:g} x V_IR, "
+                               f"{n:.3g} Q in [-{c['box']},{c['box']}]^3, FP64 fold-in/weights, FP32/c64 sums",
+                   "n_q": n, "vertices": int(T["vertices"].shape[0]), "tetrahedra": int(T["tet_volume"].size),
+                   "modes": B, "parallelism": "single GPU"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "plain": plain_res,
+        "speedup_vs_plain": (plain_res["ms_per_step"] / wall_fast) if plain_res else None,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

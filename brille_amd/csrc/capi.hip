@@ -148,14 +148,50 @@ size_t brille_last_error(char* buffer, size_t length) {
   return m.size();
 }
 
+// Every index the kernels follow is checked here, once, so that no table can
+// send a kernel outside an allocation.
+static int validate_tables(const brille_grid_desc* d) {
+  auto bad = [](const std::string& m) { return fail(BRILLE_ERR_INVALID_ARGUMENT, m); };
+  for (uint32_t j = 0; j < d->n_ops; ++j)
+    if (d->inverse_index[j] >= d->n_ops) return bad("inverse_index out of range");
+  for (uint32_t j = 0; j < d->n_faces; ++j)
+    if (!(d->prim_tau_lens[j] > 0)) return bad("first-zone tau lengths must be positive");
+  if (d->n_nodes == 0) return BRILLE_OK;
+  for (int a = 0; a < 3; ++a) {
+    if (d->n_knots[a] < 2) return bad("each trellis axis needs at least two knots");
+    for (uint32_t k = 1; k < d->n_knots[a]; ++k)
+      if (!(d->knots[a][k] > d->knots[a][k - 1])) return bad("trellis knots must increase strictly");
+  }
+  for (uint64_t i = 0; i < d->n_nodes; ++i) {
+    const uint8_t t = d->node_type[i];
+    if (t > BRILLE_NODE_POLY) return bad("unknown node type");
+    if (t == BRILLE_NODE_CUBE && d->node_index[i] >= d->n_cubes) return bad("cube node index out of range");
+    if (t == BRILLE_NODE_POLY && d->node_index[i] >= d->n_polys) return bad("poly node index out of range");
+  }
+  for (uint64_t i = 0; i < 8 * d->n_cubes; ++i)
+    if (d->cube_vertices[i] >= d->n_vertices) return bad("cube vertex index out of range");
+  if (d->n_polys) {
+    if (d->poly_tet_offset[0] != 0 || d->poly_tet_offset[d->n_polys] != d->n_tets) return bad("poly offsets do not cover the tetrahedra");
+    for (uint64_t p = 0; p < d->n_polys; ++p)
+      if (d->poly_tet_offset[p + 1] < d->poly_tet_offset[p]) return bad("poly offsets must not decrease");
+  }
+  for (uint64_t i = 0; i < 4 * d->n_tets; ++i)
+    if (d->tet_vertices[i] >= d->n_vertices) return bad("tetrahedron vertex index out of range");
+  for (uint64_t i = 0; i < d->n_tets; ++i)
+    if (!(d->tet_volume[i] > 0)) return bad("tetrahedron volumes must be positive");
+  if (d->n_tets >= 0xFFFFFFFFull || d->n_vertices >= 0xFFFFFFFFull) return bad("tables exceed 32-bit indices");
+  return BRILLE_OK;
+}
+
 int brille_grid_create(const brille_grid_desc* desc, brille_grid** out) {
   if (!desc || !out) return fail(BRILLE_ERR_INVALID_ARGUMENT, "null argument");
-  int ndev = 0;
-  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(BRILLE_ERR_NO_DEVICE, "no HIP device available");
   if (desc->n_faces < 4 || desc->n_faces > kMaxFaces)
     return fail(BRILLE_ERR_INVALID_ARGUMENT, "first Brillouin zone must have between 4 and 32 faces");
   if (desc->n_ops < 1 || desc->identity_index >= desc->n_ops)
     return fail(BRILLE_ERR_INVALID_ARGUMENT, "point group must contain the identity");
+  if (int v = validate_tables(desc); v != BRILLE_OK) return v;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(BRILLE_ERR_NO_DEVICE, "no HIP device available");
   auto* G = new brille_grid();
   GridConst& g = G->g;
   std::memset(&g, 0, sizeof(g));
@@ -304,6 +340,12 @@ int brille_grid_fill(brille_grid* G, const brille_data_desc* vals, const brille_
     if (gamma->cart_rotations) chk(upload(gamma->cart_rotations, 9 * (size_t)gamma->n_ops, &d.cart_rot, A));
   }
   if (perm && perm->n_keys) {
+    for (uint64_t i = 0; i < perm->n_keys; ++i) {
+      if (perm->perm_index[i] >= perm->n_perms) rc = fail(BRILLE_ERR_INVALID_ARGUMENT, "permutation index out of range");
+      if (i && perm->keys[i] <= perm->keys[i - 1]) rc = fail(BRILLE_ERR_INVALID_ARGUMENT, "permutation keys must be sorted");
+    }
+    for (uint64_t i = 0; i < (uint64_t)perm->n_perms * d.B; ++i)
+      if (perm->perms[i] >= d.B) rc = fail(BRILLE_ERR_INVALID_ARGUMENT, "permutation entry out of range");
     d.n_keys = perm->n_keys;
     chk(upload(perm->keys, (size_t)perm->n_keys, &d.keys, A));
     chk(upload(perm->perm_index, (size_t)perm->n_keys, &d.perm_index, A));

@@ -65,7 +65,8 @@ int check_rotation_modes(const brille_grid* G) {
   if (rc != BRILLE_OK) return rc;
   if (d.vec_rot == BRILLE_ROT_GAMMA && d.ec[1] + d.ec[2] > 0) {
     if (!d.F0) return fail(BRILLE_ERR_ROTATION, "Gamma rotation requires a GammaTable");
-    if (d.ec[1] / 3u > d.n_atoms) return fail(BRILLE_ERR_ROTATION, "more eigenvector atoms than GammaTable atoms");
+    if (d.ec[1] / 3u != d.n_atoms)
+      return fail(BRILLE_ERR_ROTATION, "the eigenvectors must hold one 3-vector per GammaTable atom");
     if (d.ec[2] > 0) return fail(BRILLE_ERR_UNSUPPORTED, "Gamma transformation of 3N x 3N tensors is not supported");
     if (d.vec_lu == BRILLE_LU_ANGSTROM && !d.cart_rot) return fail(BRILLE_ERR_ROTATION, "Cartesian Gamma rotation needs Cartesian matrices");
   }
