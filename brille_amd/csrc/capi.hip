@@ -67,13 +67,6 @@ __global__ void k_err_init(ErrRec* e) {
   }
 }
 
-template <typename Real, bool Fast>
-static void launch_interp(const brille_grid* G, const LaunchArgs& a, hipStream_t s) {
-  const uint64_t blocks = (a.n + TILE - 1) / TILE;
-  if (blocks == 0) return;
-  hipLaunchKernelGGL((k_ir_interpolate<Real, Fast>), dim3((unsigned)blocks), dim3(TILE), 0, s, G->g, G->d, a);
-}
-
 // stage (a) only
 __global__ __launch_bounds__(256) void k_ir_moveinto(GridConst g, const double* Q, uint64_t n, double* q_out,
                                                      int32_t* tau_out, uint32_t* rot_out, uint32_t* invrot_out,
@@ -96,32 +89,6 @@ __global__ __launch_bounds__(256) void k_ir_moveinto(GridConst g, const double* 
   if (st != BRILLE_PT_OK) {
     atomicMin(&err->first_bad, ((unsigned long long)(base + p) << 3) | (unsigned long long)st);
     atomicAdd(&err->count, 1ull);
-  }
-}
-
-static std::string fmt_q(const double* q) {
-  char buf[160];
-  std::snprintf(buf, sizeof(buf), "[%.17g, %.17g, %.17g]", q[0], q[1], q[2]);
-  return buf;
-}
-
-// Translate the first failing point into the reference's error text.
-static int report_point_error(const ErrRec& e, const double* Qhost_row) {
-  const uint64_t idx = e.first_bad >> 3;
-  const int st = static_cast<int>(e.first_bad & 7ull);
-  std::string q = Qhost_row ? fmt_q(Qhost_row) : ("#" + std::to_string(idx));
-  switch (st) {
-    case BRILLE_PT_OUTSIDE_BZ:
-      return fail(BRILLE_ERR_OUTSIDE_BZ, "Not all points inside Brillouin zone (first: Q = " + q + ")");
-    case BRILLE_PT_OUTSIDE_IR:
-      return fail(BRILLE_ERR_OUTSIDE_IR, "Q = " + q + " is outside of the irreducible BrillouinZone");
-    case BRILLE_PT_NULL_NODE:
-      return fail(BRILLE_ERR_NULL_NODE, "Attempting to access null node for point " + q);
-    default: {
-      std::string m = "interpolate_at failed to find " + std::to_string(e.count) + " point";
-      m += e.count > 1 ? "s." : ".";
-      return fail(BRILLE_ERR_NOT_FOUND, m);
-    }
   }
 }
 
@@ -415,16 +382,7 @@ int brille_ir_moveinto(brille_grid* G, const double* Q, uint64_t n, int32_t memo
     if (status_out) HIP_TRY(hipMemcpy(status_out, dst, n, hipMemcpyDeviceToHost));
     free_all(tmp);
   }
-  if (e.count) {
-    double row[3];
-    const uint64_t idx = e.first_bad >> 3;
-    const double* rp = nullptr;
-    if (idx < n) {
-      if (memory == BRILLE_MEM_HOST) rp = Q + 3 * idx;
-      else if (hipMemcpy(row, Q + 3 * idx, sizeof(row), hipMemcpyDeviceToHost) == hipSuccess) rp = row;
-    }
-    return report_point_error(e, rp);
-  }
+  if (e.count) return report_point_error(e, Q, n, memory == BRILLE_MEM_HOST);
   return BRILLE_OK;
 }
 

@@ -9,6 +9,10 @@
 #include <cfloat>
 #include <cstdint>
 
+// The fold-in and location routines are host-callable too, so a CPU harness
+// in tests/ can check the fast decisions against the exact ones.
+#define BRILLE_HD __host__ __device__ inline
+
 namespace brille_amd {
 
 constexpr double kTwoPiD = 6.283185307179586476925286766559;
@@ -112,16 +116,16 @@ struct Tol {
 };
 
 // approx_float::_scalar  src/approx_float.hpp:171-177
-__device__ __forceinline__ bool approx(double a, double b, Tol t) {
+BRILLE_HD bool approx(double a, double b, Tol t) {
   double x = fabs(a - b);
   return x <= t.abs + t.rel * fabs(a + b) || x < DBL_MIN;
 }
-__device__ __forceinline__ bool cmp_ge(double a, double b, Tol t) { return approx(a, b, t) || a > b; }
-__device__ __forceinline__ bool cmp_le(double a, double b, Tol t) { return approx(a, b, t) || a < b; }
-__device__ __forceinline__ bool cmp_gt(double a, double b, Tol t) { return !approx(a, b, t) && a > b; }
+BRILLE_HD bool cmp_ge(double a, double b, Tol t) { return approx(a, b, t) || a > b; }
+BRILLE_HD bool cmp_le(double a, double b, Tol t) { return approx(a, b, t) || a < b; }
+BRILLE_HD bool cmp_gt(double a, double b, Tol t) { return !approx(a, b, t) && a > b; }
 
 // utils::multiply_matrix_vector  src/utilities.tpp:44-55
-__device__ __forceinline__ void mat_vec(double* c, const double* A, const double* b) {
+BRILLE_HD void mat_vec(double* c, const double* A, const double* b) {
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
     double s = 0.0;
@@ -130,7 +134,7 @@ __device__ __forceinline__ void mat_vec(double* c, const double* A, const double
     c[i] = s;
   }
 }
-__device__ __forceinline__ void mat_vec_i(double* c, const int32_t* A, const double* b) {
+BRILLE_HD void mat_vec_i(double* c, const int32_t* A, const double* b) {
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
     double s = 0.0;
@@ -140,7 +144,7 @@ __device__ __forceinline__ void mat_vec_i(double* c, const int32_t* A, const dou
   }
 }
 // R^T b for an integer R (the wedge search's r_transpose product)
-__device__ __forceinline__ void mat_t_vec_i(double* c, const int32_t* R, const double* b) {
+BRILLE_HD void mat_t_vec_i(double* c, const int32_t* R, const double* b) {
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
     double s = 0.0;
@@ -151,7 +155,7 @@ __device__ __forceinline__ void mat_t_vec_i(double* c, const int32_t* R, const d
 }
 
 // same_lattice_dot  src/array_functions.hpp:238-246
-__device__ __forceinline__ double metric_dot(const double* x, const double* y, const double* M) {
+BRILLE_HD double metric_dot(const double* x, const double* y, const double* M) {
   double t[3];
   mat_vec(t, M, x);
   double out = 0.0;
@@ -162,7 +166,7 @@ __device__ __forceinline__ double metric_dot(const double* x, const double* y, c
 
 // lattice pseudo_orient3d  src/geometry.hpp:118-138, src/array_functions.hpp:190-202,
 // src/array_lvec_methods.tpp:53-66
-__device__ __forceinline__ double orient3d_lattice(const double* a, const double* b, const double* c,
+BRILLE_HD double orient3d_lattice(const double* a, const double* b, const double* c,
                                                    const double* d, const double* Gr, const double* Gd,
                                                    double scale) {
   double u[3], v[3], w[3], cr[3], st[3];
@@ -180,7 +184,7 @@ __device__ __forceinline__ double orient3d_lattice(const double* a, const double
 }
 
 // bare pseudo_orient3d on xyz  src/geometry.hpp:118-138
-__device__ __forceinline__ double orient3d_xyz(const double* a, const double* b, const double* c, const double* d) {
+BRILLE_HD double orient3d_xyz(const double* a, const double* b, const double* c, const double* d) {
   double u[3], v[3], w[3], cr[3];
 #pragma unroll
   for (int i = 0; i < 3; ++i) { u[i] = a[i] - d[i]; v[i] = b[i] - d[i]; w[i] = c[i] - d[i]; }
@@ -194,7 +198,7 @@ __device__ __forceinline__ double orient3d_xyz(const double* a, const double* b,
 }
 
 // point_inside_all_planes  src/geometry.hpp:413-419
-__device__ __forceinline__ bool inside_all_planes(uint32_t F, const double* pa, const double* pb, const double* pc,
+BRILLE_HD bool inside_all_planes(uint32_t F, const double* pa, const double* pb, const double* pc,
                                                   const double* x, const double* Gr, const double* Gd, double scale,
                                                   Tol t) {
   double vmin = 0.0;

@@ -9,8 +9,11 @@
 
 namespace brille_amd {
 
+// x - x is 0 exactly for every finite x and NaN for infinities and NaN
+BRILLE_HD bool finite3(const double* Q) { return Q[0] - Q[0] == 0.0 && Q[1] - Q[1] == 0.0 && Q[2] - Q[2] == 0.0; }
+
 // part_moveinto_prim for one point  src/bz_move.cpp:14-53
-__device__ inline void part_moveinto_prim(const GridConst& g, const double* Q, double* q, int32_t* tau, Tol tbz) {
+BRILLE_HD void part_moveinto_prim(const GridConst& g, const double* Q, double* q, int32_t* tau, Tol tbz) {
   const uint32_t F = g.F;
   int32_t last_shift[3];
 #pragma unroll
@@ -50,13 +53,13 @@ __device__ inline void part_moveinto_prim(const GridConst& g, const double* Q, d
   }
 }
 
-__device__ inline void to_primitive(const GridConst& g, const double* Q, double* Qp) {
+BRILLE_HD void to_primitive(const GridConst& g, const double* Q, double* Qp) {
   mat_vec_i(Qp, g.P6t, Q);
 #pragma unroll
   for (int i = 0; i < 3; ++i) Qp[i] /= 6.0;
 }
 
-__device__ inline void from_primitive(const GridConst& g, const double* qp, const int32_t* tp, double* q, int32_t* tau) {
+BRILLE_HD void from_primitive(const GridConst& g, const double* qp, const int32_t* tp, double* q, int32_t* tau) {
   mat_vec_i(q, g.invPt, qp);
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
@@ -68,8 +71,8 @@ __device__ inline void from_primitive(const GridConst& g, const double* qp, cons
 }
 
 // BrillouinZone::moveinto for one point  src/bz_move.cpp:103-163
-__device__ inline int moveinto_exact(const GridConst& g, const double* Q, double* q, int32_t* tau, Tol tbz) {
-  if (!(isfinite(Q[0]) && isfinite(Q[1]) && isfinite(Q[2]))) return BRILLE_PT_OUTSIDE_BZ;
+BRILLE_HD int moveinto_exact(const GridConst& g, const double* Q, double* q, int32_t* tau, Tol tbz) {
+  if (!finite3(Q)) return BRILLE_PT_OUTSIDE_BZ;
   double Qp[3], qp[3];
   int32_t tp[3];
   if (g.transform_needed) to_primitive(g, Q, Qp);
@@ -87,7 +90,7 @@ __device__ inline int moveinto_exact(const GridConst& g, const double* Q, double
 }
 
 // BrillouinZone::_inside_wedge_outer  src/bz.hpp:757-763
-__device__ inline bool inside_wedge(const GridConst& g, const double* p, Tol tbz, Tol tdef) {
+BRILLE_HD bool inside_wedge(const GridConst& g, const double* p, Tol tbz, Tol tdef) {
   const uint32_t W = g.W;
   if (W == 0) return true;
   if (g.no_ir_mirroring) {
@@ -111,7 +114,7 @@ __device__ inline bool inside_wedge(const GridConst& g, const double* p, Tol tbz
 }
 
 // ordered wedge search  src/bz_move.cpp:262-285
-__device__ inline int wedge_search_exact(const GridConst& g, double* q, uint32_t* rot, uint32_t* invrot, Tol tbz, Tol tdef) {
+BRILLE_HD int wedge_search_exact(const GridConst& g, double* q, uint32_t* rot, uint32_t* invrot, Tol tbz, Tol tdef) {
   if (inside_wedge(g, q, tbz, tdef)) {
     *rot = *invrot = g.identity;
     return BRILLE_PT_OK;
@@ -130,7 +133,7 @@ __device__ inline int wedge_search_exact(const GridConst& g, double* q, uint32_t
 }
 
 // find_bin / on_boundary  src/trellis_poly.hpp:66-81 (linear scan over the stored knots)
-__device__ inline uint32_t find_bin_scan(const double* edges, uint32_t n, double x) {
+BRILLE_HD uint32_t find_bin_scan(const double* edges, uint32_t n, double x) {
   uint32_t d = 0;
   while (d < n && !(edges[d] > x)) ++d;
   if (d > n - 1 && x < edges[0]) d = 0;
@@ -139,7 +142,7 @@ __device__ inline uint32_t find_bin_scan(const double* edges, uint32_t n, double
 
 // the same answer from O(1) arithmetic: guess floor((x - k0)/step) and move
 // to the first stored knot > x, exactly as the scan defines it
-__device__ inline uint32_t find_bin_fast(const double* edges, uint32_t n, double lo, double inv_step, double x) {
+BRILLE_HD uint32_t find_bin_fast(const double* edges, uint32_t n, double lo, double inv_step, double x) {
   if (!(x >= edges[0])) return find_bin_scan(edges, n, x);  // NaN or below the first knot
   double f = fmin((x - lo) * inv_step, (double)n);
   int64_t d = (int64_t)f + 1;  // candidate index of the first knot > x
@@ -150,13 +153,13 @@ __device__ inline uint32_t find_bin_fast(const double* edges, uint32_t n, double
   return d > 0 ? (uint32_t)(d - 1) : 0u;
 }
 
-__device__ inline int on_boundary(const double* edges, uint32_t n, double x, uint32_t i, Tol tdef) {
+BRILLE_HD int on_boundary(const double* edges, uint32_t n, double x, uint32_t i, Tol tdef) {
   if (i + 2 < n && approx(edges[i + 1], x, tdef)) return 1;
   if (i > 0 && approx(edges[i], x, tdef)) return -1;
   return 0;
 }
 
-__device__ inline bool node_usable(const GridConst& g, const int64_t* sub, uint64_t* idx) {
+BRILLE_HD bool node_usable(const GridConst& g, const int64_t* sub, uint64_t* idx) {
 #pragma unroll
   for (int d = 0; d < 3; ++d)
     if (sub[d] < 0 || sub[d] >= (int64_t)(g.nk[d] - 1)) return false;
@@ -169,7 +172,7 @@ __device__ inline bool node_usable(const GridConst& g, const int64_t* sub, uint6
 
 // PolyTrellis::node_subscript  src/trellis_poly.hpp:382-434
 template <bool FastBins>
-__device__ inline bool node_subscript(const GridConst& g, const double* x, uint64_t* node, Tol tdef) {
+BRILLE_HD bool node_subscript(const GridConst& g, const double* x, uint64_t* node, Tol tdef) {
   int64_t sub[3], ns[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d)
@@ -212,7 +215,7 @@ __device__ inline bool node_subscript(const GridConst& g, const double* x, uint6
 }
 
 // CubeNode::indices_weights  src/trellis_node.hpp:132-149
-__device__ inline int cube_weights(const GridConst& g, uint32_t c, const double* x, uint32_t* idx, double* w, Tol tdef) {
+BRILLE_HD int cube_weights(const GridConst& g, uint32_t c, const double* x, uint32_t* idx, double* w, Tol tdef) {
   const uint32_t* vi = g.cube_v + 8 * (uint64_t)c;
   uint32_t v[8];
 #pragma unroll
@@ -240,7 +243,7 @@ __device__ inline int cube_weights(const GridConst& g, uint32_t c, const double*
 }
 
 // PolyNode::tetrahedra_might_contain  src/trellis_node.hpp:349-364
-__device__ inline double tet_might_contain(const GridConst& g, uint64_t t, const double* x, Tol tdef) {
+BRILLE_HD double tet_might_contain(const GridConst& g, uint64_t t, const double* x, Tol tdef) {
   const double* ci = g.tet_c + 4 * t;
   double v0 = ci[0] - x[0], v1 = ci[1] - x[1], v2 = ci[2] - x[2];
   double d2 = 0.0, r2 = ci[3] * ci[3];
@@ -251,7 +254,7 @@ __device__ inline double tet_might_contain(const GridConst& g, uint64_t t, const
 }
 
 // PolyNode::tetrahedra_contains  src/trellis_node.hpp:319-339
-__device__ inline double tet_contains(const GridConst& g, uint64_t t, const double* x, double* w, bool shortcut, Tol tdef) {
+BRILLE_HD double tet_contains(const GridConst& g, uint64_t t, const double* x, double* w, bool shortcut, Tol tdef) {
   if (shortcut) {
     double away = tet_might_contain(g, t, x, tdef);
     if (away < 0.) return away;
@@ -282,7 +285,7 @@ __device__ inline double tet_contains(const GridConst& g, uint64_t t, const doub
   return 0.;
 }
 
-__device__ inline int keep_weights(const GridConst& g, uint64_t t, const double* ww, uint32_t* idx, double* w, Tol tdef) {
+BRILLE_HD int keep_weights(const GridConst& g, uint64_t t, const double* ww, uint32_t* idx, double* w, Tol tdef) {
   int n = 0;
 #pragma unroll
   for (int j = 0; j < 4; ++j)
@@ -295,7 +298,7 @@ __device__ inline int keep_weights(const GridConst& g, uint64_t t, const double*
 }
 
 // PolyNode::indices_weights, should_contain=true  src/trellis_node.hpp:275-308
-__device__ inline int poly_weights(const GridConst& g, uint32_t p, const double* x, uint32_t* idx, double* w,
+BRILLE_HD int poly_weights(const GridConst& g, uint32_t p, const double* x, uint32_t* idx, double* w,
                                    uint32_t* tet_out, Tol tdef) {
   const uint32_t t0 = g.poly_off[p], t1 = g.poly_off[p + 1];
   double ww[4];

@@ -34,6 +34,8 @@ int build_fast_tables(const brille_grid_desc* desc, brille_grid* G);
 void free_proposal(brille_grid* G);
 void free_staging(brille_grid* G);
 int check_rotation_modes(const brille_grid* G);
+// the first failing point of a call as the reference's error text
+int report_point_error(const ErrRec& e, const double* Q, uint64_t n, bool host_mem);
 int run_interpolate(brille_grid* G, const double* Q, uint64_t n, const brille_run_opts* opts, void* values_out,
                     void* vectors_out, const brille_aux_out* aux, hipStream_t s, brille_kernel_times* times,
                     uint32_t iters);

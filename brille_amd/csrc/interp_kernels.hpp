@@ -363,6 +363,91 @@ __device__ inline void gamma_phases(const DataConst& d, const LocRec& r, typenam
   for (uint32_t k = 0; k < n_atoms; ++k) ph[k] = gamma_phase_one<Real>(d, r, k);
 }
 
+// Payload layouts with a specialised phase 2 (the general one handles all).
+enum Layout : int {
+  LAYOUT_GENERIC = 0,  // any elements, rotations and permutation table
+  LAYOUT_GAMMA = 1,    // scalar values; vectors = one 3-vector per atom rotating like Gamma; no permutations
+  LAYOUT_SCALAR = 2    // scalar values and scalar vectors; no permutations
+};
+
+// Phase 2, specialised: scalar values of one (point, mode) unit.
+template <typename Real>
+__device__ __forceinline__ void unit_values_scalar(const DataConst& d, const LocRec& r, uint32_t b, Real* out) {
+  const uint32_t sv = d.sv;
+  Real* o = out + (uint64_t)b * sv;
+  for (uint32_t e = 0; e < sv; ++e) {
+    Real acc = 0;
+    for (int x = 0; x < r.n; ++x) acc += (Real)r.w[x] * load_val<Real>(d, ((uint64_t)r.idx[x] * d.B + b) * sv + e);
+    o[e] = acc;
+  }
+}
+
+// Phase 2, specialised: antiphased Gamma-rotated eigenvectors of one (point, mode) unit.
+template <typename Real>
+__device__ __forceinline__ void unit_vectors_gamma(const GridConst& g, const DataConst& d, const LocRec& r, uint32_t b,
+                                                   typename Cplx<Real>::T* out, const typename Cplx<Real>::T* phases,
+                                                   bool cart) {
+  using C = typename Cplx<Real>::T;
+  const uint32_t sc = d.sc, nv = sc / 3u;
+  const uint64_t pb = ((uint64_t)r.idx[0] * d.B + b) * sc;
+  C cw[8];
+  const int n = r.n;
+  for (int x = 0; x < n; ++x) {
+    C e;
+    if (x == 0) {
+      // the pivot against itself: a positive real Hermitian product, e^{-i0} = 1
+      e.x = 1;
+      e.y = 0;
+    } else {
+      e = antiphase<Real>(d, pb, ((uint64_t)r.idx[x] * d.B + b) * sc, sc);
+    }
+    Real w = (Real)r.w[x];
+    cw[x].x = w * e.x;
+    cw[x].y = w * e.y;
+  }
+  C* o = out + (uint64_t)b * sc;
+  const int32_t* iR = g.rot + 9 * (uint64_t)r.invrot;
+  const double* cR = cart ? d.cart_rot + 9 * (uint64_t)r.invrot : nullptr;
+  for (uint32_t k = 0; k < nv; ++k) {
+    C v[3] = {Elem<C>::zero(), Elem<C>::zero(), Elem<C>::zero()};
+    for (int x = 0; x < n; ++x) {
+      const uint64_t base = ((uint64_t)r.idx[x] * d.B + b) * sc + 3u * k;
+#pragma unroll
+      for (int j = 0; j < 3; ++j) v[j] = Elem<C>::add(v[j], cmul(cw[x], load_vec<Real>(d, base + j)));
+    }
+    C t[3];
+    if (cart) rot3<C, double, Real>(t, cR, v);
+    else rot3<C, int32_t, Real>(t, iR, v);
+    const uint32_t l = d.F0[k * d.gamma_ops + r.invrot];
+    C ph = phases ? phases[k] : gamma_phase_one<Real>(d, r, k);
+    C* dst = o + 3u * l;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) dst[j] = cmul(ph, t[j]);
+  }
+}
+
+// Phase 2, specialised: scalar (unrotated) vectors of one (point, mode) unit.
+template <typename Real>
+__device__ __forceinline__ void unit_vectors_scalar(const DataConst& d, const LocRec& r, uint32_t b,
+                                                    typename Cplx<Real>::T* out) {
+  using C = typename Cplx<Real>::T;
+  const uint32_t sc = d.sc;
+  const uint64_t pb = ((uint64_t)r.idx[0] * d.B + b) * sc;
+  C cw[8];
+  for (int x = 0; x < r.n; ++x) {
+    C e = antiphase<Real>(d, pb, ((uint64_t)r.idx[x] * d.B + b) * sc, sc);
+    Real w = (Real)r.w[x];
+    cw[x].x = w * e.x;
+    cw[x].y = w * e.y;
+  }
+  C* o = out + (uint64_t)b * sc;
+  for (uint32_t e = 0; e < sc; ++e) {
+    C acc = Elem<C>::zero();
+    for (int x = 0; x < r.n; ++x) acc = Elem<C>::add(acc, cmul(cw[x], load_vec<Real>(d, ((uint64_t)r.idx[x] * d.B + b) * sc + e)));
+    o[e] = acc;
+  }
+}
+
 struct LaunchArgs {
   const double* Q;
   uint64_t n;
@@ -408,7 +493,7 @@ __device__ inline void locate_point(const GridConst& g, const double* Q, bool no
   if (nw < 1) r.status = BRILLE_PT_NOT_FOUND;
 }
 
-template <typename Real, bool Fast>
+template <typename Real, bool Fast, int L>
 __global__ __launch_bounds__(TILE) void k_ir_interpolate(GridConst g, DataConst d, LaunchArgs a) {
   using C = typename Cplx<Real>::T;
   __shared__ LocRec s_rec[TILE];
@@ -467,9 +552,17 @@ __global__ __launch_bounds__(TILE) void k_ir_interpolate(GridConst g, DataConst 
       for (uint32_t e = 0; e < d.sc; ++e) oc[e] = Elem<C>::zero();
       continue;
     }
-    unit_values<Real>(g, d, r, b, vals + pt * B * d.sv);
     const C* ph = (want_phase && ph_in_lds) ? s_ph + 4 * pl : nullptr;
-    unit_vectors<Real>(g, d, r, b, vecs + pt * B * d.sc, ph);
+    if constexpr (L == LAYOUT_GAMMA) {
+      unit_values_scalar<Real>(d, r, b, vals + pt * B * d.sv);
+      unit_vectors_gamma<Real>(g, d, r, b, vecs + pt * B * d.sc, ph, d.vec_lu == BRILLE_LU_ANGSTROM);
+    } else if constexpr (L == LAYOUT_SCALAR) {
+      unit_values_scalar<Real>(d, r, b, vals + pt * B * d.sv);
+      unit_vectors_scalar<Real>(d, r, b, vecs + pt * B * d.sc);
+    } else {
+      unit_values<Real>(g, d, r, b, vals + pt * B * d.sv);
+      unit_vectors<Real>(g, d, r, b, vecs + pt * B * d.sc, ph);
+    }
   }
 }
 

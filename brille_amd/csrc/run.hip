@@ -7,6 +7,7 @@
 #include <string>
 #include <vector>
 
+#include "fast_tables.hpp"
 #include "grid_state.hpp"
 
 using namespace brille_amd;
@@ -73,24 +74,41 @@ int check_rotation_modes(const brille_grid* G) {
   return BRILLE_OK;
 }
 
-template <typename Real, bool Fast>
+template <typename Real, bool Fast, int L>
 static void launch(const brille_grid* G, const LaunchArgs& a, hipStream_t s) {
   const uint64_t blocks = (a.n + TILE - 1) / TILE;
   if (!blocks) return;
-  hipLaunchKernelGGL((k_ir_interpolate<Real, Fast>), dim3((unsigned)blocks), dim3(TILE), 0, s, G->g, G->d, a);
+  hipLaunchKernelGGL((k_ir_interpolate<Real, Fast, L>), dim3((unsigned)blocks), dim3(TILE), 0, s, G->g, G->d, a);
 }
 
+// which specialised phase 2 the payload admits
+static int payload_layout(const DataConst& d) {
+  const bool scalar_values = d.ev[1] == 0 && d.ev[2] == 0;
+  if (d.n_keys || !scalar_values) return LAYOUT_GENERIC;
+  if (d.ec[1] == 0 && d.ec[2] == 0) return LAYOUT_SCALAR;
+  if (d.ec[0] == 0 && d.ec[2] == 0 && d.vec_rot == BRILLE_ROT_GAMMA &&
+      (d.vec_lu == BRILLE_LU_REAL_LATTICE || d.vec_lu == BRILLE_LU_ANGSTROM))
+    return LAYOUT_GAMMA;
+  return LAYOUT_GENERIC;
+}
+
+// plain: the literal FP64 restatement; tuned: fast fold-in and specialised sums
 static void launch_sel(const brille_grid* G, const LaunchArgs& a, int precision, bool fast, hipStream_t s) {
+  const int L = payload_layout(G->d);
   if (precision == BRILLE_FP32) {
-    if (fast) launch<float, true>(G, a, s);
-    else launch<float, false>(G, a, s);
+    if (!fast) launch<float, false, LAYOUT_GENERIC>(G, a, s);
+    else if (L == LAYOUT_GAMMA) launch<float, true, LAYOUT_GAMMA>(G, a, s);
+    else if (L == LAYOUT_SCALAR) launch<float, true, LAYOUT_SCALAR>(G, a, s);
+    else launch<float, true, LAYOUT_GENERIC>(G, a, s);
   } else {
-    if (fast) launch<double, true>(G, a, s);
-    else launch<double, false>(G, a, s);
+    if (!fast) launch<double, false, LAYOUT_GENERIC>(G, a, s);
+    else if (L == LAYOUT_GAMMA) launch<double, true, LAYOUT_GAMMA>(G, a, s);
+    else if (L == LAYOUT_SCALAR) launch<double, true, LAYOUT_SCALAR>(G, a, s);
+    else launch<double, true, LAYOUT_GENERIC>(G, a, s);
   }
 }
 
-static int report(const brille_grid* G, const ErrRec& e, const double* Q, uint64_t n, bool host_mem) {
+int report_point_error(const ErrRec& e, const double* Q, uint64_t n, bool host_mem) {
   const uint64_t idx = e.first_bad >> 3;
   const int st = (int)(e.first_bad & 7ull);
   double row[3];
@@ -105,7 +123,6 @@ static int report(const brille_grid* G, const ErrRec& e, const double* Q, uint64
     std::snprintf(buf, sizeof(buf), "[%.17g, %.17g, %.17g]", rp[0], rp[1], rp[2]);
     q = buf;
   }
-  (void)G;
   switch (st) {
     case BRILLE_PT_OUTSIDE_BZ:
       return fail(BRILLE_ERR_OUTSIDE_BZ, "Not all points inside Brillouin zone (first: Q = " + q + ")");
@@ -182,7 +199,7 @@ int run_interpolate(brille_grid* G, const double* Q, uint64_t n, const brille_ru
     ErrRec e;
     HIP_TRY(hipMemcpyAsync(&e, G->err, sizeof(e), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
-    if (e.count) return report(G, e, Q, n, false);
+    if (e.count) return report_point_error(e, Q, n, false);
     return BRILLE_OK;
   }
   // host memory: stream through device staging in chunks
@@ -236,13 +253,37 @@ int run_interpolate(brille_grid* G, const double* Q, uint64_t n, const brille_ru
   ErrRec e;
   HIP_TRY(hipMemcpyAsync(&e, G->err, sizeof(e), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
-  if (e.count) return report(G, e, Q, n, true);
+  if (e.count) return report_point_error(e, Q, n, true);
   return BRILLE_OK;
 }
 
-int build_fast_tables(const brille_grid_desc* desc, brille_grid* G) {
-  (void)desc;
-  (void)G;
+// Affine forms of the zone-face orientation tests and the seed orbit of the
+// wedge (see fast_fold.hpp).  pseudo_orient3d(a,b,c,x) in a lattice with
+// reciprocal volume V equals V det[a-x, b-x, c-x] on the components, i.e.
+// V det[a,b,c] - x . V (a x b + b x c + c x a).
+int build_fast_tables(const brille_grid_desc* d, brille_grid* G) {
+  GridConst& g = G->g;
+  FastTables ft = compute_fast_tables(d);
+  g.aff_margin = ft.aff_margin;
+  auto up = [&](const std::vector<double>& v, const double** dst) -> int {
+    void* p = nullptr;
+    if (hipMalloc(&p, v.size() * sizeof(double)) != hipSuccess) return fail(BRILLE_ERR_DEVICE, "hipMalloc failed");
+    G->grid_allocs.push_back(p);
+    if (hipMemcpy(p, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+      return fail(BRILLE_ERR_DEVICE, "hipMemcpy failed");
+    *dst = static_cast<const double*>(p);
+    return BRILLE_OK;
+  };
+  int rc = up(ft.prim_aff, &g.prim_aff);
+  if (rc == BRILLE_OK) rc = up(ft.outer_aff, &g.outer_aff);
+  if (rc != BRILLE_OK) return rc;
+  if (ft.has_orbit) {
+    rc = up(ft.orbit, &g.orbit);
+    if (rc != BRILLE_OK) return rc;
+    g.orbit_margin = ft.orbit_margin;
+  } else {
+    g.has_seed = 0;
+  }
   return BRILLE_OK;
 }
 

@@ -99,7 +99,10 @@ def test_fp32_path_indices_exact_values_within_1e5(cfg, vf):
     assert rc == 0 and grc == 0
     for k in EXACT:
         assert np.array_equal(oaux[k], gaux[k]), k
-    rel_v = np.max(np.abs(gv - ov) / np.maximum(np.abs(ov), 1e-30))
+    # eigenvalues: error relative to each mode's eigenvalue scale (the synthetic cosine series
+    # passes through zero, where a pointwise ratio is meaningless)
+    scale = np.max(np.abs(ov), axis=0, keepdims=True)
+    rel_v = np.max(np.abs(gv - ov) / scale)
     assert rel_v <= 1e-5, rel_v
     # eigenvectors: error relative to the norm of each mode's vector
     nrm = np.sqrt(np.sum(np.abs(oc.reshape(oc.shape[0], oc.shape[1], -1)) ** 2, axis=2))
