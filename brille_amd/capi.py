@@ -350,6 +350,8 @@ class Grid:
         return self.data.vdesc.branches
 
     def out_shapes(self, n):
+        if self.data is None:  # unfilled: the library reports it before touching the outputs
+            return (n, 0), (n, 0)
         vs = (n,) + self.data.vals.shape[1:]
         cs = (n,) + self.data.vecs.shape[1:]
         return vs, cs

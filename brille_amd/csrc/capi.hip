@@ -49,6 +49,32 @@ static void free_all(std::vector<void*>& allocs) {
   allocs.clear();
 }
 
+// reals per payload row: 16-byte multiple, and a power of two up to a cache
+// line so that no row of 128 bytes or less straddles two lines
+static uint32_t row_stride(uint32_t used, size_t real_bytes) {
+  size_t bytes = ((size_t)used * real_bytes + 15) / 16 * 16;
+  if (bytes <= 128) {
+    size_t p = 16;
+    while (p < bytes) p <<= 1;
+    bytes = p;
+  }
+  return (uint32_t)(bytes / real_bytes);
+}
+
+// rows of (vertex, branch): vectors (re, im interleaved) then values, zero pad
+static void build_rows(const DataConst& d, const double* vals, const double* vecs, uint32_t stride,
+                       std::vector<double>& out) {
+  const size_t R = (size_t)d.V * d.B;
+  out.assign(R * stride, 0.0);
+  for (size_t r = 0; r < R; ++r) {
+    double* o = out.data() + r * stride;
+    const double* c = vecs + r * 2 * d.sc;
+    for (uint32_t k = 0; k < 2 * d.sc; ++k) o[k] = c[k];
+    const double* v = vals + r * d.sv;
+    for (uint32_t k = 0; k < d.sv; ++k) o[2 * d.sc + k] = v[k];
+  }
+}
+
 static void make_tols(double Ttol, int tol, double* rel, double* abs_) {
   // approx_float::tols  src/approx_float.hpp:79-100
   double Trel = std::numeric_limits<double>::epsilon();
@@ -286,17 +312,23 @@ int brille_grid_fill(brille_grid* G, const brille_data_desc* vals, const brille_
   d.vec_rot = vecs->rotates_like;
   d.vec_lu = vecs->length_unit;
   auto& A = G->data_allocs;
-  const size_t nv = (size_t)d.V * d.B * d.sv, nc = (size_t)d.V * d.B * d.sc;
   int rc = BRILLE_OK;
   auto chk = [&](int r) { if (r != BRILLE_OK && rc == BRILLE_OK) rc = r; };
-  chk(upload(vals->data, nv, &d.vals64, A));
-  chk(upload(reinterpret_cast<const double2*>(vecs->data), nc, &d.vecs64, A));
-  std::vector<float> f32(nv);
-  for (size_t i = 0; i < nv; ++i) f32[i] = static_cast<float>(vals->data[i]);
-  chk(upload(f32.data(), nv, &d.vals32, A));
-  std::vector<float2> c32(nc);
-  for (size_t i = 0; i < nc; ++i) c32[i] = make_float2((float)vecs->data[2 * i], (float)vecs->data[2 * i + 1]);
-  chk(upload(c32.data(), nc, &d.vecs32, A));
+  const uint32_t used = 2 * d.sc + d.sv;
+  d.stride64 = row_stride(used, sizeof(double));
+  d.stride32 = row_stride(used, sizeof(float));
+  {
+    std::vector<double> r64;
+    build_rows(d, vals->data, vecs->data, d.stride64, r64);
+    chk(upload(r64.data(), r64.size(), &d.rows64, A));
+  }
+  {
+    std::vector<double> r;
+    build_rows(d, vals->data, vecs->data, d.stride32, r);
+    std::vector<float> r32(r.size());
+    for (size_t i = 0; i < r.size(); ++i) r32[i] = static_cast<float>(r[i]);
+    chk(upload(r32.data(), r32.size(), &d.rows32, A));
+  }
   if (gamma) {
     if (gamma->n_ops != G->g.G) rc = fail(BRILLE_ERR_INVALID_ARGUMENT, "GammaTable and point group sizes differ");
     d.n_atoms = gamma->n_atoms;

@@ -80,10 +80,14 @@ struct DataConst {
   uint32_t sv, sc;        // span of values / vectors per branch
   uint32_t ev[3], ec[3];  // element counts (scalars, vector elements, matrix elements)
   int32_t val_rot, val_lu, vec_rot, vec_lu;
-  const double* vals64;
-  const double2* vecs64;
-  const float* vals32;
-  const float2* vecs32;
+  // One row per (vertex, branch): the sc vector elements (complex,
+  // interleaved re/im) followed by the sv value elements, zero-padded to a
+  // multiple of 16 bytes (to 32/64/128 bytes when the row fits in a cache
+  // line), so a row never needs more than its own lines and loads 16 bytes at
+  // a time.  Row (v, b) starts at rows + (v * B + b) * stride.
+  const double* rows64;
+  const float* rows32;
+  uint32_t stride64, stride32;  // in reals
   // GammaTable
   uint32_t n_atoms, gamma_ops;
   const uint32_t* F0;

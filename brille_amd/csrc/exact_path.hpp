@@ -214,8 +214,11 @@ BRILLE_HD bool node_subscript(const GridConst& g, const double* x, uint64_t* nod
   return false;
 }
 
-// CubeNode::indices_weights  src/trellis_node.hpp:132-149
-BRILLE_HD int cube_weights(const GridConst& g, uint32_t c, const double* x, uint32_t* idx, double* w, Tol tdef) {
+// CubeNode::indices_weights  src/trellis_node.hpp:132-149, in fixed slots:
+// slot i holds vertex 7-i and its weight.  A slot the reference drops (weight
+// not > 0) keeps weight 0, so the non-zero slots in slot order are exactly the
+// reference's (index, weight) list.
+BRILLE_HD void cube_slots(const GridConst& g, uint32_t c, const double* x, uint32_t* idx, double* w, Tol tdef) {
   const uint32_t* vi = g.cube_v + 8 * (uint64_t)c;
   uint32_t v[8];
 #pragma unroll
@@ -225,7 +228,6 @@ BRILLE_HD int cube_weights(const GridConst& g, uint32_t c, const double* x, uint
   double node_volume = 1.0;
 #pragma unroll
   for (int k = 0; k < 3; ++k) node_volume *= fabs(v0[k] - v7[k]);
-  int n = 0;
 #pragma unroll
   for (int i = 0; i < 8; ++i) {
     const double* vv = g.verts + 3 * (uint64_t)v[i];
@@ -233,13 +235,9 @@ BRILLE_HD int cube_weights(const GridConst& g, uint32_t c, const double* x, uint
 #pragma unroll
     for (int k = 0; k < 3; ++k) p *= fabs(x[k] - vv[k]);
     double wi = p / node_volume;
-    if (cmp_gt(wi, 0.0, tdef)) {
-      idx[n] = v[7 - i];
-      w[n] = wi;
-      ++n;
-    }
+    idx[i] = v[7 - i];
+    w[i] = cmp_gt(wi, 0.0, tdef) ? wi : 0.0;
   }
-  return n;
 }
 
 // PolyNode::tetrahedra_might_contain  src/trellis_node.hpp:349-364
@@ -285,21 +283,19 @@ BRILLE_HD double tet_contains(const GridConst& g, uint64_t t, const double* x, d
   return 0.;
 }
 
-BRILLE_HD int keep_weights(const GridConst& g, uint64_t t, const double* ww, uint32_t* idx, double* w, Tol tdef) {
-  int n = 0;
+// slot j of tetrahedron t; weight 0 where the reference drops the vertex
+// (approximately zero weight, src/trellis_node.hpp:300-305)
+BRILLE_HD void tet_slots(const GridConst& g, uint64_t t, const double* ww, uint32_t* idx, double* w, Tol tdef) {
 #pragma unroll
-  for (int j = 0; j < 4; ++j)
-    if (!approx(ww[j], 0., tdef)) {
-      idx[n] = g.tet_v[4 * t + j];
-      w[n] = ww[j];
-      ++n;
-    }
-  return n;
+  for (int j = 0; j < 4; ++j) {
+    idx[j] = g.tet_v[4 * t + j];
+    w[j] = approx(ww[j], 0., tdef) ? 0.0 : ww[j];
+  }
 }
 
-// PolyNode::indices_weights, should_contain=true  src/trellis_node.hpp:275-308
-BRILLE_HD int poly_weights(const GridConst& g, uint32_t p, const double* x, uint32_t* idx, double* w,
-                                   uint32_t* tet_out, Tol tdef) {
+// PolyNode::indices_weights, should_contain=true  src/trellis_node.hpp:275-308.
+// Fills slots 0..3 and returns the tetrahedron used (kNoTet: node has none).
+BRILLE_HD uint32_t poly_slots(const GridConst& g, uint32_t p, const double* x, uint32_t* idx, double* w, Tol tdef) {
   const uint32_t t0 = g.poly_off[p], t1 = g.poly_off[p + 1];
   double ww[4];
   double best = 0.0;
@@ -307,15 +303,15 @@ BRILLE_HD int poly_weights(const GridConst& g, uint32_t p, const double* x, uint
   for (uint32_t t = t0; t < t1; ++t) {
     double mn = tet_contains(g, t, x, ww, true, tdef);
     if (mn >= 0.) {
-      *tet_out = t;
-      return keep_weights(g, t, ww, idx, w, tdef);
+      tet_slots(g, t, ww, idx, w, tdef);
+      return t;
     }
     if (t == t0 || mn > best) { best = mn; best_t = t; }
   }
-  if (t1 == t0) return 0;
+  if (t1 == t0) return kNoTet;
   tet_contains(g, best_t, x, ww, false, tdef);
-  *tet_out = best_t;
-  return keep_weights(g, best_t, ww, idx, w, tdef);
+  tet_slots(g, best_t, ww, idx, w, tdef);
+  return best_t;
 }
 
 }  // namespace brille_amd

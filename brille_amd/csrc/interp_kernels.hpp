@@ -2,17 +2,23 @@
 //
 // One workgroup owns a tile of TILE points.
 //   phase 1  (one lane per point): fold-in (stage a) and point location
-//            (stage b) in FP64; the (index, weight) pairs, Ridx/invRidx and
-//            the irreducible q go to LDS.
-//   phase 2  (one lane per (point, mode) unit, consecutive lanes take
-//            consecutive modes of a point so loads of vertex rows and stores
-//            of output rows coalesce): weighted sums of values and of
-//            antiphased vectors (stage c1) fused with the symmetry rotation
-//            (stage c2), written straight to the output rows.
+//            (stage b) in FP64, in registers; the slots (vertex, weight), the
+//            antiphase pivot and Ridx/invRidx go to LDS as a PointRec.
+//   phase 2  (one lane per (point, mode) unit; consecutive lanes take
+//            consecutive modes of a point, whose payload rows are adjacent):
+//            weighted sums of values and of antiphased vectors (stage c1)
+//            fused with the symmetry rotation (stage c2), written straight to
+//            the output rows.
+// Slots: a cube node fills 8 (slot i = vertex 7-i), a tetrahedron 4.  A slot
+// the reference drops carries weight 0, so the non-zero slots in order are the
+// reference's (index, weight) list and every loop over slots has a fixed trip
+// count (no runtime-indexed arrays, hence no scratch memory).
 // Real = double reproduces the reference's FP64 arithmetic; Real = float runs
 // the sums and the rotation in FP32/complex64 with the same FP64 indices and
 // weights.
 #pragma once
+#include <type_traits>
+
 #include "exact_path.hpp"
 #include "fast_fold.hpp"
 
@@ -25,10 +31,12 @@ struct Cplx;
 template <>
 struct Cplx<double> {
   using T = double2;
+  using V16 = double2;  // one 16-byte load
 };
 template <>
 struct Cplx<float> {
   using T = float2;
+  using V16 = float4;
 };
 
 struct ErrRec {
@@ -36,14 +44,67 @@ struct ErrRec {
   unsigned long long count;
 };
 
-struct LocRec {
+// Phase 1 result of one point, held in registers.
+struct Located {
   double q[3];
+  int32_t tau[3];
+  uint32_t rot, invrot;
+  uint32_t node, tet;
   uint32_t idx[8];
   double w[8];
-  uint32_t rot, invrot;
-  uint8_t n;
-  uint8_t status;
+  uint32_t pivot;  // first kept vertex: the antiphase reference of the sums
+  int nslot;       // 0 (failed), 4 or 8
+  int nkept;
+  int status;
 };
+
+// What phase 2 needs of a point, in LDS.
+template <typename Real>
+struct PointRec {
+  uint32_t idx[8];
+  Real w[8];
+  uint32_t pivot;
+  uint16_t rot, invrot;
+  uint8_t nslot, pad0, pad1, pad2;
+};
+
+// ---------------------------------------------------------------------------
+// payload rows
+
+template <typename Real>
+__device__ __forceinline__ const Real* payload_rows(const DataConst& d);
+template <>
+__device__ __forceinline__ const double* payload_rows<double>(const DataConst& d) { return d.rows64; }
+template <>
+__device__ __forceinline__ const float* payload_rows<float>(const DataConst& d) { return d.rows32; }
+template <typename Real>
+__device__ __forceinline__ uint32_t payload_stride(const DataConst& d) {
+  return sizeof(Real) == 8 ? d.stride64 : d.stride32;
+}
+template <typename Real>
+__device__ __forceinline__ const Real* row_at(const DataConst& d, uint32_t v, uint32_t b) {
+  return payload_rows<Real>(d) + ((uint64_t)v * d.B + b) * payload_stride<Real>(d);
+}
+
+// the first N reals of a 16-byte aligned, 16-byte padded row
+template <typename Real, int N>
+__device__ __forceinline__ void load_row(Real* out, const Real* row) {
+  using V = typename Cplx<Real>::V16;
+  constexpr int PER = 16 / (int)sizeof(Real);
+  constexpr int CH = (N + PER - 1) / PER;
+  const V* p = reinterpret_cast<const V*>(row);
+#pragma unroll
+  for (int c = 0; c < CH; ++c) {
+    union {
+      V v;
+      Real r[PER];
+    } u;
+    u.v = p[c];
+#pragma unroll
+    for (int j = 0; j < PER; ++j)
+      if (c * PER + j < N) out[c * PER + j] = u.r[j];
+  }
+}
 
 // PermutationTable::safe_get with pivot iw[0]  src/permutation_table.hpp:193-198
 __device__ __forceinline__ uint32_t perm_at(const DataConst& d, uint32_t pivot, uint32_t idx, uint32_t b) {
@@ -59,51 +120,60 @@ __device__ __forceinline__ uint32_t perm_at(const DataConst& d, uint32_t pivot, 
   return b;
 }
 
-template <typename Real>
-__device__ __forceinline__ Real load_val(const DataConst& d, uint64_t i);
-template <>
-__device__ __forceinline__ double load_val<double>(const DataConst& d, uint64_t i) { return d.vals64[i]; }
-template <>
-__device__ __forceinline__ float load_val<float>(const DataConst& d, uint64_t i) { return d.vals32[i]; }
+// ---------------------------------------------------------------------------
+// antiphase
 
-template <typename Real>
-__device__ __forceinline__ typename Cplx<Real>::T load_vec(const DataConst& d, uint64_t i);
-template <>
-__device__ __forceinline__ double2 load_vec<double>(const DataConst& d, uint64_t i) { return d.vecs64[i]; }
-template <>
-__device__ __forceinline__ float2 load_vec<float>(const DataConst& d, uint64_t i) { return d.vecs32[i]; }
-
-// utils::antiphase  src/utilities.tpp:567-577: e^{-i theta} of the Hermitian
-// product of a and b.  FP64 keeps the reference's atan2/cos/sin; FP32 uses
-// the identical closed form conj(z)/|z| (cos(atan2(y,x)) = x/|z|,
-// sin(atan2(y,x)) = y/|z|) which needs no transcendental.
-template <typename Real>
-__device__ __forceinline__ typename Cplx<Real>::T antiphase(const DataConst& d, uint64_t a, uint64_t b, uint32_t n);
-template <>
-__device__ __forceinline__ double2 antiphase<double>(const DataConst& d, uint64_t a, uint64_t b, uint32_t n) {
-  double re = 0, im = 0;
-  for (uint32_t i = 0; i < n; ++i) {
-    double2 x = d.vecs64[a + i], y = d.vecs64[b + i];
-    re += x.x * y.x + x.y * y.y;
-    im += x.x * y.y - x.y * y.x;
-  }
+// utils::antiphase  src/utilities.tpp:567-577 of the Hermitian product
+// z = re + i im: polar(1, -atan2(im, re)).  Literal keeps the reference's
+// atan2/cos/sin; otherwise the identical closed form conj(z)/|z| (cos(atan2(y,
+// x)) = x/|z|, sin(atan2(y,x)) = y/|z|), scaled so |z|^2 neither underflows nor
+// overflows, with the two cases atan2 fixes exactly kept exact: z real and
+// positive gives 1, z = 0 gives +-1 by the sign of re.
+// one out-of-line copy: the slot loops that call it are unrolled, and eight
+// inlined atan2/sincos bodies would crowd the instruction cache
+__device__ __noinline__ double2 antiphase_literal(double re, double im) {
   double th = -1.0 * atan2(im, re);
   double s, c;
   sincos(th, &s, &c);
   return make_double2(1.0 * c, 1.0 * s);
 }
-template <>
-__device__ __forceinline__ float2 antiphase<float>(const DataConst& d, uint64_t a, uint64_t b, uint32_t n) {
-  float re = 0.f, im = 0.f;
-  for (uint32_t i = 0; i < n; ++i) {
-    float2 x = d.vecs32[a + i], y = d.vecs32[b + i];
-    re += x.x * y.x + x.y * y.y;
-    im += x.x * y.y - x.y * y.x;
+
+template <typename Real, bool Literal>
+__device__ __forceinline__ typename Cplx<Real>::T antiphase_of(Real re, Real im) {
+  typename Cplx<Real>::T e;
+  if constexpr (Literal) {
+    const double2 t = antiphase_literal((double)re, (double)im);
+    e.x = (Real)t.x;
+    e.y = (Real)t.y;
+  } else {
+    const Real m = fmax(fabs(re), fabs(im));
+    if (im == (Real)0 && re > (Real)0) {
+      e.x = 1;
+      e.y = 0;
+    } else if (m > (Real)0) {
+      const Real a = re / m, b = im / m;
+      const Real inv = (Real)1 / sqrt(a * a + b * b);
+      e.x = a * inv;
+      e.y = -b * inv;
+    } else {
+      e.x = signbit(re) ? (Real)-1 : (Real)1;
+      e.y = 0;
+    }
   }
-  float r2 = re * re + im * im;
-  if (!(r2 > 0.f)) return make_float2(re < 0.f ? -1.f : 1.f, 0.f);
-  float inv = rsqrtf(r2);
-  return make_float2(re * inv, -im * inv);
+  return e;
+}
+
+// Hermitian product sum_k conj(p_k) x_k over n complex elements, in the
+// reference's order (re += p.re x.re + p.im x.im; im += p.re x.im - p.im x.re)
+template <typename Real>
+__device__ __forceinline__ void hermitian(const Real* p, const Real* x, uint32_t n, Real* re, Real* im) {
+  Real a = 0, b = 0;
+  for (uint32_t k = 0; k < n; ++k) {
+    a += p[2 * k] * x[2 * k] + p[2 * k + 1] * x[2 * k + 1];
+    b += p[2 * k] * x[2 * k + 1] - p[2 * k + 1] * x[2 * k];
+  }
+  *re = a;
+  *im = b;
 }
 
 template <typename C>
@@ -233,54 +303,15 @@ __device__ __forceinline__ void rotate_mat3(int rot_kind, int lu, const GridCons
   }
 }
 
-// Phase 2 for one (point, mode) unit: values.
+// e^{i 2 pi q . d} for atom k of a point  src/interpolator_gamma.tpp:18-32
 template <typename Real>
-__device__ inline void unit_values(const GridConst& g, const DataConst& d, const LocRec& r, uint32_t b, Real* out) {
-  const uint32_t sv = d.sv, ns = d.ev[0], nv = d.ev[1] / 3u, nm = d.ev[2] / 9u;
-  const uint32_t pivot = r.idx[0];
-  uint64_t base[8];
-  Real w[8];
-  for (int x = 0; x < r.n; ++x) {
-    base[x] = ((uint64_t)r.idx[x] * d.B + perm_at(d, pivot, r.idx[x], b)) * sv;
-    w[x] = (Real)r.w[x];
-  }
-  Real* o = out + (uint64_t)b * sv;
-  uint32_t e = 0;
-  for (; e < ns; ++e) {
-    Real acc = 0;
-    for (int x = 0; x < r.n; ++x) acc += w[x] * load_val<Real>(d, base[x] + e);
-    o[e] = acc;
-  }
-  for (uint32_t k = 0; k < nv; ++k, e += 3) {
-    Real v[3] = {0, 0, 0};
-    for (int x = 0; x < r.n; ++x)
-#pragma unroll
-      for (int j = 0; j < 3; ++j) v[j] += w[x] * load_val<Real>(d, base[x] + e + j);
-    rotate_vec3<Real, Real>(d.val_rot, d.val_lu, g, r.rot, r.invrot, v);
-    o[e] = v[0]; o[e + 1] = v[1]; o[e + 2] = v[2];
-  }
-  for (uint32_t k = 0; k < nm; ++k, e += 9) {
-    Real m[9];
-#pragma unroll
-    for (int j = 0; j < 9; ++j) m[j] = 0;
-    for (int x = 0; x < r.n; ++x)
-#pragma unroll
-      for (int j = 0; j < 9; ++j) m[j] += w[x] * load_val<Real>(d, base[x] + e + j);
-    rotate_mat3<Real, Real>(d.val_rot, d.val_lu, g, r.rot, r.invrot, m);
-#pragma unroll
-    for (int j = 0; j < 9; ++j) o[e + j] = m[j];
-  }
-}
-
-// e_iqd for one atom of one point  src/interpolator_gamma.tpp:18-32
-template <typename Real>
-__device__ inline typename Cplx<Real>::T gamma_phase_one(const DataConst& d, const LocRec& r, uint32_t k) {
-  const uint32_t key = k * d.gamma_ops + r.invrot;
+__device__ inline typename Cplx<Real>::T gamma_phase(const DataConst& d, const double* q, uint32_t invrot, uint32_t k) {
+  const uint32_t key = k * d.gamma_ops + invrot;
   const double* dv = d.gvec + 3 * (uint64_t)d.vidx[key];
   double dot = 0;
-  dot += r.q[0] * dv[0];
-  dot += r.q[1] * dv[1];
-  dot += r.q[2] * dv[2];
+  dot += q[0] * dv[0];
+  dot += q[1] * dv[1];
+  dot += q[2] * dv[2];
   double arg = kTwoPiD * dot;
   typename Cplx<Real>::T ph;
   if constexpr (sizeof(Real) == 8) {
@@ -297,70 +328,196 @@ __device__ inline typename Cplx<Real>::T gamma_phase_one(const DataConst& d, con
   return ph;
 }
 
-// Phase 2 for one (point, mode) unit: antiphased vectors + rotation.
-template <typename Real>
-__device__ inline void unit_vectors(const GridConst& g, const DataConst& d, const LocRec& r, uint32_t b,
-                                    typename Cplx<Real>::T* out, const typename Cplx<Real>::T* phases) {
+// ---------------------------------------------------------------------------
+// phase 2, general payload: any elements, rotations, GammaTable and
+// PermutationTable; two passes over the rows (antiphases, then sums).
+
+template <typename Real, bool Literal>
+__device__ inline void unit_generic(const GridConst& g, const DataConst& d, const PointRec<Real>& r, const double* q,
+                                    uint32_t b, Real* ov, typename Cplx<Real>::T* oc,
+                                    const typename Cplx<Real>::T* phases) {
   using C = typename Cplx<Real>::T;
-  const uint32_t sc = d.sc, ns = d.ec[0], nv = d.ec[1] / 3u, nm = d.ec[2] / 9u;
-  const uint32_t pivot = r.idx[0];
-  uint64_t base[8];
+  const uint32_t sv = d.sv, sc = d.sc;
+  const uint32_t ns = d.ev[0], nv = d.ev[1] / 3u, nm = d.ev[2] / 9u;
+  const int nslot = r.nslot;
+  const Real* prow = row_at<Real>(d, r.pivot, b);
+  const Real* row[8];
+  Real w[8];
   C cw[8];
-  const uint64_t pbase = ((uint64_t)pivot * d.B + b) * sc;
-  for (int x = 0; x < r.n; ++x) {
-    base[x] = ((uint64_t)r.idx[x] * d.B + perm_at(d, pivot, r.idx[x], b)) * sc;
-    C e = antiphase<Real>(d, pbase, base[x], sc);
-    Real w = (Real)r.w[x];
-    cw[x].x = w * e.x;  // weights[x]*eith
-    cw[x].y = w * e.y;
+#pragma unroll
+  for (int x = 0; x < 8; ++x) {
+    const bool use = x < nslot && r.w[x] != (Real)0;
+    w[x] = use ? r.w[x] : (Real)0;
+    row[x] = use ? row_at<Real>(d, r.idx[x], perm_at(d, r.pivot, r.idx[x], b)) : prow;
+    cw[x] = Elem<C>::zero();
+    if (use && sc) {
+      Real re, im;
+      hermitian<Real>(prow, row[x], sc, &re, &im);
+      C e = antiphase_of<Real, Literal>(re, im);
+      cw[x].x = w[x] * e.x;  // weights[x]*eith
+      cw[x].y = w[x] * e.y;
+    }
   }
-  C* o = out + (uint64_t)b * sc;
+  // values  (interpolate_at_mix, real part)
   uint32_t e = 0;
   for (; e < ns; ++e) {
+    Real acc = 0;
+#pragma unroll
+    for (int x = 0; x < 8; ++x)
+      if (w[x] != (Real)0) acc += w[x] * row[x][2 * sc + e];
+    ov[e] = acc;
+  }
+  for (uint32_t k = 0; k < nv; ++k, e += 3) {
+    Real v[3] = {0, 0, 0};
+#pragma unroll
+    for (int x = 0; x < 8; ++x)
+      if (w[x] != (Real)0)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) v[j] += w[x] * row[x][2 * sc + e + j];
+    rotate_vec3<Real, Real>(d.val_rot, d.val_lu, g, r.rot, r.invrot, v);
+    ov[e] = v[0]; ov[e + 1] = v[1]; ov[e + 2] = v[2];
+  }
+  for (uint32_t k = 0; k < nm; ++k, e += 9) {
+    Real m[9];
+#pragma unroll
+    for (int j = 0; j < 9; ++j) m[j] = 0;
+#pragma unroll
+    for (int x = 0; x < 8; ++x)
+      if (w[x] != (Real)0)
+#pragma unroll
+        for (int j = 0; j < 9; ++j) m[j] += w[x] * row[x][2 * sc + e + j];
+    rotate_mat3<Real, Real>(d.val_rot, d.val_lu, g, r.rot, r.invrot, m);
+#pragma unroll
+    for (int j = 0; j < 9; ++j) ov[e + j] = m[j];
+  }
+  // vectors (interpolate_at_mix, complex part)
+  const uint32_t cs = d.ec[0], cv = d.ec[1] / 3u, cm = d.ec[2] / 9u;
+  auto vec_at = [&](int x, uint32_t el) {
+    C c;
+    c.x = row[x][2 * el];
+    c.y = row[x][2 * el + 1];
+    return c;
+  };
+  e = 0;
+  for (; e < cs; ++e) {
     C acc = Elem<C>::zero();
-    for (int x = 0; x < r.n; ++x) acc = Elem<C>::add(acc, cmul(cw[x], load_vec<Real>(d, base[x] + e)));
-    o[e] = acc;
+#pragma unroll
+    for (int x = 0; x < 8; ++x)
+      if (w[x] != (Real)0) acc = Elem<C>::add(acc, cmul(cw[x], vec_at(x, e)));
+    oc[e] = acc;
   }
   const bool gamma = d.vec_rot == BRILLE_ROT_GAMMA;
   const bool cart = d.vec_lu == BRILLE_LU_ANGSTROM;
-  for (uint32_t k = 0; k < nv; ++k, e += 3) {
+  for (uint32_t k = 0; k < cv; ++k, e += 3) {
     C v[3] = {Elem<C>::zero(), Elem<C>::zero(), Elem<C>::zero()};
-    for (int x = 0; x < r.n; ++x)
 #pragma unroll
-      for (int j = 0; j < 3; ++j) v[j] = Elem<C>::add(v[j], cmul(cw[x], load_vec<Real>(d, base[x] + e + j)));
+    for (int x = 0; x < 8; ++x)
+      if (w[x] != (Real)0)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) v[j] = Elem<C>::add(v[j], cmul(cw[x], vec_at(x, e + j)));
     if (gamma) {
       // rip_gamma_complex  src/interpolator_gamma.tpp:100-114
       C t[3];
       if (cart) rot3<C, double, Real>(t, d.cart_rot + 9 * (uint64_t)r.invrot, v);
       else rot3<C, int32_t, Real>(t, g.rot + 9 * (uint64_t)r.invrot, v);
-      const uint32_t key = k * d.gamma_ops + r.invrot;
-      const uint32_t l = d.F0[key];
-      C ph = phases ? phases[k] : gamma_phase_one<Real>(d, r, k);
-      C* dst = o + ns + 3u * l;
+      const uint32_t l = d.F0[k * d.gamma_ops + r.invrot];
+      C ph = phases ? phases[k] : gamma_phase<Real>(d, q, r.invrot, k);
+      C* dst = oc + cs + 3u * l;
 #pragma unroll
       for (int j = 0; j < 3; ++j) dst[j] = cmul(ph, t[j]);
     } else {
       rotate_vec3<C, Real>(d.vec_rot, d.vec_lu, g, r.rot, r.invrot, v);
-      o[e] = v[0]; o[e + 1] = v[1]; o[e + 2] = v[2];
+      oc[e] = v[0]; oc[e + 1] = v[1]; oc[e + 2] = v[2];
     }
   }
-  for (uint32_t k = 0; k < nm; ++k, e += 9) {
+  for (uint32_t k = 0; k < cm; ++k, e += 9) {
     C m[9];
 #pragma unroll
     for (int j = 0; j < 9; ++j) m[j] = Elem<C>::zero();
-    for (int x = 0; x < r.n; ++x)
 #pragma unroll
-      for (int j = 0; j < 9; ++j) m[j] = Elem<C>::add(m[j], cmul(cw[x], load_vec<Real>(d, base[x] + e + j)));
+    for (int x = 0; x < 8; ++x)
+      if (w[x] != (Real)0)
+#pragma unroll
+        for (int j = 0; j < 9; ++j) m[j] = Elem<C>::add(m[j], cmul(cw[x], vec_at(x, e + j)));
     rotate_mat3<C, Real>(d.vec_rot, d.vec_lu, g, r.rot, r.invrot, m);
 #pragma unroll
-    for (int j = 0; j < 9; ++j) o[e + j] = m[j];
+    for (int j = 0; j < 9; ++j) oc[e + j] = m[j];
   }
 }
 
-// e_iqd for every atom of one point
-template <typename Real>
-__device__ inline void gamma_phases(const DataConst& d, const LocRec& r, typename Cplx<Real>::T* ph, uint32_t n_atoms) {
-  for (uint32_t k = 0; k < n_atoms; ++k) ph[k] = gamma_phase_one<Real>(d, r, k);
+// ---------------------------------------------------------------------------
+// phase 2, specialised payloads with compile-time row sizes: SC complex vector
+// elements and SV scalar values per (vertex, mode), no PermutationTable.
+//   GAMMA: the vectors are SC/3 atom 3-vectors rotating like Gamma
+//          (rip_gamma_complex  src/interpolator_gamma.tpp:100-114);
+//   otherwise the vectors are scalars (no rotation).
+// One pass: every row is read once, with 16-byte loads, and contributes its
+// antiphase and its weighted elements before the next row is read.
+
+template <typename Real, int SC, int SV, bool GAMMA>
+__device__ __forceinline__ void unit_onepass(const GridConst& g, const DataConst& d, const PointRec<Real>& r,
+                                             uint32_t b, Real* ov, typename Cplx<Real>::T* oc,
+                                             const typename Cplx<Real>::T* phases, bool cart) {
+  using C = typename Cplx<Real>::T;
+  constexpr int NR = 2 * SC + SV;
+  const Real* base = payload_rows<Real>(d);
+  const uint64_t stride = payload_stride<Real>(d);
+  Real pr[2 * SC];
+  load_row<Real, 2 * SC>(pr, base + ((uint64_t)r.pivot * d.B + b) * stride);
+  Real acc[2 * SC];
+  Real vacc[SV];
+#pragma unroll
+  for (int k = 0; k < 2 * SC; ++k) acc[k] = 0;
+#pragma unroll
+  for (int s = 0; s < SV; ++s) vacc[s] = 0;
+  const int nslot = r.nslot;
+#pragma unroll
+  for (int x = 0; x < 8; ++x) {
+    if (x == 4 && nslot <= 4) break;
+    const Real w = r.w[x];
+    Real row[NR];
+    load_row<Real, NR>(row, base + ((uint64_t)r.idx[x] * d.B + b) * stride);
+    Real re = 0, im = 0;
+#pragma unroll
+    for (int k = 0; k < SC; ++k) {
+      re += pr[2 * k] * row[2 * k] + pr[2 * k + 1] * row[2 * k + 1];
+      im += pr[2 * k] * row[2 * k + 1] - pr[2 * k + 1] * row[2 * k];
+    }
+    const C e = antiphase_of<Real, false>(re, im);
+    const Real cx = w * e.x, cy = w * e.y;
+#pragma unroll
+    for (int k = 0; k < SC; ++k) {
+      acc[2 * k] += cx * row[2 * k] - cy * row[2 * k + 1];
+      acc[2 * k + 1] += cx * row[2 * k + 1] + cy * row[2 * k];
+    }
+#pragma unroll
+    for (int s = 0; s < SV; ++s) vacc[s] += w * row[2 * SC + s];
+  }
+#pragma unroll
+  for (int s = 0; s < SV; ++s) ov[s] = vacc[s];
+  if constexpr (GAMMA) {
+    const int32_t* iR = g.rot + 9 * (uint64_t)r.invrot;
+    const double* cR = cart ? d.cart_rot + 9 * (uint64_t)r.invrot : nullptr;
+#pragma unroll
+    for (int a = 0; a < SC / 3; ++a) {
+      C v[3], t[3];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        v[j].x = acc[2 * (3 * a + j)];
+        v[j].y = acc[2 * (3 * a + j) + 1];
+      }
+      if (cart) rot3<C, double, Real>(t, cR, v);
+      else rot3<C, int32_t, Real>(t, iR, v);
+      const uint32_t l = d.F0[a * d.gamma_ops + r.invrot];
+      const C ph = phases[a];
+      C* dst = oc + 3u * l;
+#pragma unroll
+      for (int j = 0; j < 3; ++j) dst[j] = cmul(ph, t[j]);
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < SC; ++k) oc[k] = C{acc[2 * k], acc[2 * k + 1]};
+  }
 }
 
 // Payload layouts with a specialised phase 2 (the general one handles all).
@@ -369,84 +526,6 @@ enum Layout : int {
   LAYOUT_GAMMA = 1,    // scalar values; vectors = one 3-vector per atom rotating like Gamma; no permutations
   LAYOUT_SCALAR = 2    // scalar values and scalar vectors; no permutations
 };
-
-// Phase 2, specialised: scalar values of one (point, mode) unit.
-template <typename Real>
-__device__ __forceinline__ void unit_values_scalar(const DataConst& d, const LocRec& r, uint32_t b, Real* out) {
-  const uint32_t sv = d.sv;
-  Real* o = out + (uint64_t)b * sv;
-  for (uint32_t e = 0; e < sv; ++e) {
-    Real acc = 0;
-    for (int x = 0; x < r.n; ++x) acc += (Real)r.w[x] * load_val<Real>(d, ((uint64_t)r.idx[x] * d.B + b) * sv + e);
-    o[e] = acc;
-  }
-}
-
-// Phase 2, specialised: antiphased Gamma-rotated eigenvectors of one (point, mode) unit.
-template <typename Real>
-__device__ __forceinline__ void unit_vectors_gamma(const GridConst& g, const DataConst& d, const LocRec& r, uint32_t b,
-                                                   typename Cplx<Real>::T* out, const typename Cplx<Real>::T* phases,
-                                                   bool cart) {
-  using C = typename Cplx<Real>::T;
-  const uint32_t sc = d.sc, nv = sc / 3u;
-  const uint64_t pb = ((uint64_t)r.idx[0] * d.B + b) * sc;
-  C cw[8];
-  const int n = r.n;
-  for (int x = 0; x < n; ++x) {
-    C e;
-    if (x == 0) {
-      // the pivot against itself: a positive real Hermitian product, e^{-i0} = 1
-      e.x = 1;
-      e.y = 0;
-    } else {
-      e = antiphase<Real>(d, pb, ((uint64_t)r.idx[x] * d.B + b) * sc, sc);
-    }
-    Real w = (Real)r.w[x];
-    cw[x].x = w * e.x;
-    cw[x].y = w * e.y;
-  }
-  C* o = out + (uint64_t)b * sc;
-  const int32_t* iR = g.rot + 9 * (uint64_t)r.invrot;
-  const double* cR = cart ? d.cart_rot + 9 * (uint64_t)r.invrot : nullptr;
-  for (uint32_t k = 0; k < nv; ++k) {
-    C v[3] = {Elem<C>::zero(), Elem<C>::zero(), Elem<C>::zero()};
-    for (int x = 0; x < n; ++x) {
-      const uint64_t base = ((uint64_t)r.idx[x] * d.B + b) * sc + 3u * k;
-#pragma unroll
-      for (int j = 0; j < 3; ++j) v[j] = Elem<C>::add(v[j], cmul(cw[x], load_vec<Real>(d, base + j)));
-    }
-    C t[3];
-    if (cart) rot3<C, double, Real>(t, cR, v);
-    else rot3<C, int32_t, Real>(t, iR, v);
-    const uint32_t l = d.F0[k * d.gamma_ops + r.invrot];
-    C ph = phases ? phases[k] : gamma_phase_one<Real>(d, r, k);
-    C* dst = o + 3u * l;
-#pragma unroll
-    for (int j = 0; j < 3; ++j) dst[j] = cmul(ph, t[j]);
-  }
-}
-
-// Phase 2, specialised: scalar (unrotated) vectors of one (point, mode) unit.
-template <typename Real>
-__device__ __forceinline__ void unit_vectors_scalar(const DataConst& d, const LocRec& r, uint32_t b,
-                                                    typename Cplx<Real>::T* out) {
-  using C = typename Cplx<Real>::T;
-  const uint32_t sc = d.sc;
-  const uint64_t pb = ((uint64_t)r.idx[0] * d.B + b) * sc;
-  C cw[8];
-  for (int x = 0; x < r.n; ++x) {
-    C e = antiphase<Real>(d, pb, ((uint64_t)r.idx[x] * d.B + b) * sc, sc);
-    Real w = (Real)r.w[x];
-    cw[x].x = w * e.x;
-    cw[x].y = w * e.y;
-  }
-  C* o = out + (uint64_t)b * sc;
-  for (uint32_t e = 0; e < sc; ++e) {
-    C acc = Elem<C>::zero();
-    for (int x = 0; x < r.n; ++x) acc = Elem<C>::add(acc, cmul(cw[x], load_vec<Real>(d, ((uint64_t)r.idx[x] * d.B + b) * sc + e)));
-    o[e] = acc;
-  }
-}
 
 struct LaunchArgs {
   const double* Q;
@@ -460,78 +539,136 @@ struct LaunchArgs {
   int use_proposal;
 };
 
+// ---------------------------------------------------------------------------
 // phase 1 for one point
+
 template <bool Fast>
-__device__ inline void locate_point(const GridConst& g, const double* Q, bool no_move, LocRec& r, uint32_t* node_out,
-                                    uint32_t* tet_out, int32_t* tau_out) {
+__device__ inline void locate(const GridConst& g, const double* Q, bool no_move, Located& L) {
   const Tol tbz{g.tol_bz_rel, g.tol_bz_abs};
   const Tol tdef{g.tol_def_rel, g.tol_def_abs};
-  int32_t tau[3] = {0, 0, 0};
-  r.n = 0;
-  r.status = BRILLE_PT_OK;
-  r.rot = r.invrot = g.identity;
-  *node_out = 0xFFFFFFFFu;
-  *tet_out = kNoTet;
-  if (no_move) {
-    r.q[0] = Q[0]; r.q[1] = Q[1]; r.q[2] = Q[2];
-  } else {
-    int st = Fast ? fast_moveinto(g, Q, r.q, tau, tbz) : moveinto_exact(g, Q, r.q, tau, tbz);
-    if (st == BRILLE_PT_OK) st = Fast ? fast_wedge_search(g, r.q, &r.rot, &r.invrot, tbz, tdef)
-                                      : wedge_search_exact(g, r.q, &r.rot, &r.invrot, tbz, tdef);
-    if (st != BRILLE_PT_OK) { r.status = (uint8_t)st; tau_out[0] = tau[0]; tau_out[1] = tau[1]; tau_out[2] = tau[2]; return; }
+  L.tau[0] = L.tau[1] = L.tau[2] = 0;
+  L.rot = L.invrot = g.identity;
+  L.node = 0xFFFFFFFFu;
+  L.tet = kNoTet;
+  L.nslot = 0;
+  L.nkept = 0;
+  L.pivot = 0;
+  L.status = BRILLE_PT_OK;
+#pragma unroll
+  for (int s = 0; s < 8; ++s) {
+    L.idx[s] = 0;
+    L.w[s] = 0.0;
   }
-  tau_out[0] = tau[0]; tau_out[1] = tau[1]; tau_out[2] = tau[2];
+  if (no_move) {
+    L.q[0] = Q[0]; L.q[1] = Q[1]; L.q[2] = Q[2];
+  } else {
+    int st = Fast ? fast_moveinto(g, Q, L.q, L.tau, tbz) : moveinto_exact(g, Q, L.q, L.tau, tbz);
+    if (st == BRILLE_PT_OK) st = Fast ? fast_wedge_search(g, L.q, &L.rot, &L.invrot, tbz, tdef)
+                                      : wedge_search_exact(g, L.q, &L.rot, &L.invrot, tbz, tdef);
+    if (st != BRILLE_PT_OK) { L.status = st; return; }
+  }
   double x[3];
-  mat_vec(x, g.outer_to_xyz, r.q);  // brille::get_xyz
+  mat_vec(x, g.outer_to_xyz, L.q);  // brille::get_xyz
   uint64_t node = 0;
-  if (!node_subscript<Fast>(g, x, &node, tdef)) { r.status = BRILLE_PT_NULL_NODE; return; }
-  *node_out = (uint32_t)node;
-  int nw;
-  if (g.node_type[node] == BRILLE_NODE_CUBE) nw = cube_weights(g, g.node_index[node], x, r.idx, r.w, tdef);
-  else nw = poly_weights(g, g.node_index[node], x, r.idx, r.w, tet_out, tdef);
-  r.n = (uint8_t)nw;
-  if (nw < 1) r.status = BRILLE_PT_NOT_FOUND;
+  if (!node_subscript<Fast>(g, x, &node, tdef)) { L.status = BRILLE_PT_NULL_NODE; return; }
+  L.node = (uint32_t)node;
+  if (g.node_type[node] == BRILLE_NODE_CUBE) {
+    cube_slots(g, g.node_index[node], x, L.idx, L.w, tdef);
+    L.nslot = 8;
+  } else {
+    L.tet = poly_slots(g, g.node_index[node], x, L.idx, L.w, tdef);
+    L.nslot = L.tet == kNoTet ? 0 : 4;
+  }
+  // pivot = first kept slot; unused tetrahedron slots repeat it with weight 0
+#pragma unroll
+  for (int s = 7; s >= 0; --s)
+    if (s < L.nslot && L.w[s] != 0.0) L.pivot = L.idx[s];
+#pragma unroll
+  for (int s = 0; s < 8; ++s) {
+    if (s < L.nslot && L.w[s] != 0.0) ++L.nkept;
+    if (s >= L.nslot) L.idx[s] = L.pivot;
+  }
+  if (L.nkept < 1) { L.status = BRILLE_PT_NOT_FOUND; L.nslot = 0; }
 }
 
-template <typename Real, bool Fast, int L>
+// aux outputs of one point (weights in FP64 whatever the sums' precision)
+__device__ inline void write_aux(const brille_aux_out& aux, uint64_t p, const Located& L) {
+  if (aux.ir_q) { aux.ir_q[3 * p] = L.q[0]; aux.ir_q[3 * p + 1] = L.q[1]; aux.ir_q[3 * p + 2] = L.q[2]; }
+  if (aux.tau) { aux.tau[3 * p] = L.tau[0]; aux.tau[3 * p + 1] = L.tau[1]; aux.tau[3 * p + 2] = L.tau[2]; }
+  if (aux.rot) aux.rot[p] = L.rot;
+  if (aux.invrot) aux.invrot[p] = L.invrot;
+  if (aux.node) aux.node[p] = L.node;
+  if (aux.tet) aux.tet[p] = L.tet;
+  if (aux.n_iw) aux.n_iw[p] = (uint8_t)(L.status == BRILLE_PT_OK ? L.nkept : 0);
+  if (aux.iw_index || aux.iw_weight) {
+    uint32_t k = 0;
+#pragma unroll
+    for (int s = 0; s < 8; ++s)
+      if (L.status == BRILLE_PT_OK && s < L.nslot && L.w[s] != 0.0) {
+        if (aux.iw_index) aux.iw_index[8 * p + k] = L.idx[s];
+        if (aux.iw_weight) aux.iw_weight[8 * p + k] = L.w[s];
+        ++k;
+      }
+    for (; k < 8; ++k) {
+      if (aux.iw_index) aux.iw_index[8 * p + k] = 0u;
+      if (aux.iw_weight) aux.iw_weight[8 * p + k] = 0.0;
+    }
+  }
+  if (aux.status) aux.status[p] = (uint8_t)L.status;
+}
+
+// ---------------------------------------------------------------------------
+// the kernel.  L selects phase 2; SC/SV are the compile-time row sizes of the
+// specialised layouts (ignored by LAYOUT_GENERIC).  Plain (Fast = false) is
+// the literal restatement: exact fold-in and location, atan2/sincos
+// antiphases.
+
+template <typename Real, bool Fast, int L, int SC, int SV>
 __global__ __launch_bounds__(TILE) void k_ir_interpolate(GridConst g, DataConst d, LaunchArgs a) {
   using C = typename Cplx<Real>::T;
-  __shared__ LocRec s_rec[TILE];
-  __shared__ C s_ph[TILE * 4];
+  constexpr int NPH = L == LAYOUT_GAMMA ? SC / 3 : (L == LAYOUT_GENERIC ? 4 : 1);
+  constexpr int NQ = L == LAYOUT_GENERIC ? 3 * TILE : 1;
+  __shared__ PointRec<Real> s_rec[TILE];
+  __shared__ C s_ph[TILE * NPH];
+  __shared__ double s_q[NQ];
   const uint64_t tile0 = (uint64_t)blockIdx.x * TILE;
   const uint32_t tid = threadIdx.x;
   const uint64_t p = tile0 + tid;
   const bool want_phase = d.vec_rot == BRILLE_ROT_GAMMA && d.ec[1] > 0;
   const uint32_t n_atoms = d.ec[1] / 3u;
-  const bool ph_in_lds = want_phase && n_atoms <= 4;
-  if (p < a.n) {
-    LocRec& r = s_rec[tid];
-    double Qp[3] = {a.Q[3 * p], a.Q[3 * p + 1], a.Q[3 * p + 2]};
-    uint32_t node, tet;
-    int32_t tau[3];
-    locate_point<Fast>(g, Qp, a.no_move, r, &node, &tet, tau);
-    if (a.aux.ir_q) { a.aux.ir_q[3 * p] = r.q[0]; a.aux.ir_q[3 * p + 1] = r.q[1]; a.aux.ir_q[3 * p + 2] = r.q[2]; }
-    if (a.aux.tau) { a.aux.tau[3 * p] = tau[0]; a.aux.tau[3 * p + 1] = tau[1]; a.aux.tau[3 * p + 2] = tau[2]; }
-    if (a.aux.rot) a.aux.rot[p] = r.rot;
-    if (a.aux.invrot) a.aux.invrot[p] = r.invrot;
-    if (a.aux.node) a.aux.node[p] = node;
-    if (a.aux.tet) a.aux.tet[p] = tet;
-    if (a.aux.n_iw) a.aux.n_iw[p] = r.n;
-    if (a.aux.iw_index)
-      for (int k = 0; k < 8; ++k) a.aux.iw_index[8 * p + k] = k < r.n ? r.idx[k] : 0u;
-    if (a.aux.iw_weight)
-      for (int k = 0; k < 8; ++k) a.aux.iw_weight[8 * p + k] = k < r.n ? r.w[k] : 0.0;
-    if (a.aux.status) a.aux.status[p] = r.status;
-    if (r.status != BRILLE_PT_OK) {
-      r.n = 0;
-      unsigned long long code = ((unsigned long long)(a.base + p) << 3) | r.status;
-      atomicMin(&a.err->first_bad, code);
-      atomicAdd(&a.err->count, 1ull);
-    } else if (ph_in_lds) {
-      gamma_phases<Real>(d, r, s_ph + 4 * tid, n_atoms);
+  const bool ph_in_lds = L == LAYOUT_GAMMA || (want_phase && n_atoms <= 4);
+  {
+    PointRec<Real>& r = s_rec[tid];
+    r.nslot = 0;
+    if (p < a.n) {
+      double Qp[3] = {a.Q[3 * p], a.Q[3 * p + 1], a.Q[3 * p + 2]};
+      Located P;
+      locate<Fast>(g, Qp, a.no_move, P);
+      write_aux(a.aux, p, P);
+      if (P.status != BRILLE_PT_OK) {
+        unsigned long long code = ((unsigned long long)(a.base + p) << 3) | (unsigned long long)P.status;
+        atomicMin(&a.err->first_bad, code);
+        atomicAdd(&a.err->count, 1ull);
+      } else {
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+          r.idx[s] = P.idx[s];
+          r.w[s] = (Real)P.w[s];
+        }
+        r.pivot = P.pivot;
+        r.rot = (uint16_t)P.rot;
+        r.invrot = (uint16_t)P.invrot;
+        r.nslot = (uint8_t)P.nslot;
+        if constexpr (L == LAYOUT_GAMMA) {
+#pragma unroll
+          for (int k = 0; k < NPH; ++k) s_ph[NPH * tid + k] = gamma_phase<Real>(d, P.q, P.invrot, k);
+        } else if constexpr (L == LAYOUT_GENERIC) {
+          s_q[3 * tid] = P.q[0]; s_q[3 * tid + 1] = P.q[1]; s_q[3 * tid + 2] = P.q[2];
+          if (ph_in_lds)
+            for (uint32_t k = 0; k < n_atoms; ++k) s_ph[NPH * tid + k] = gamma_phase<Real>(d, P.q, P.invrot, k);
+        }
+      }
     }
-  } else {
-    s_rec[tid].n = 0;
   }
   __syncthreads();
   const uint64_t left = a.n - tile0;
@@ -540,28 +677,25 @@ __global__ __launch_bounds__(TILE) void k_ir_interpolate(GridConst g, DataConst 
   const uint32_t units = npts * B;
   Real* vals = (Real*)a.vals;
   C* vecs = (C*)a.vecs;
+  const bool cart = d.vec_lu == BRILLE_LU_ANGSTROM;
   for (uint32_t u = tid; u < units; u += TILE) {
     const uint32_t pl = u / B, b = u - pl * B;
-    const LocRec& r = s_rec[pl];
-    const uint64_t pt = tile0 + pl;
-    if (r.n == 0) {
+    const PointRec<Real>& r = s_rec[pl];
+    const uint64_t row = (tile0 + pl) * B + b;
+    Real* ov = vals + row * d.sv;
+    C* oc = vecs + row * d.sc;
+    if (r.nslot == 0) {
       // failed points: zero rows (the call reports the error)
-      Real* ov = vals + (pt * B + b) * d.sv;
       for (uint32_t e = 0; e < d.sv; ++e) ov[e] = 0;
-      C* oc = vecs + (pt * B + b) * d.sc;
       for (uint32_t e = 0; e < d.sc; ++e) oc[e] = Elem<C>::zero();
       continue;
     }
-    const C* ph = (want_phase && ph_in_lds) ? s_ph + 4 * pl : nullptr;
     if constexpr (L == LAYOUT_GAMMA) {
-      unit_values_scalar<Real>(d, r, b, vals + pt * B * d.sv);
-      unit_vectors_gamma<Real>(g, d, r, b, vecs + pt * B * d.sc, ph, d.vec_lu == BRILLE_LU_ANGSTROM);
+      unit_onepass<Real, SC, SV, true>(g, d, r, b, ov, oc, s_ph + NPH * pl, cart);
     } else if constexpr (L == LAYOUT_SCALAR) {
-      unit_values_scalar<Real>(d, r, b, vals + pt * B * d.sv);
-      unit_vectors_scalar<Real>(d, r, b, vecs + pt * B * d.sc);
+      unit_onepass<Real, SC, SV, false>(g, d, r, b, ov, oc, nullptr, cart);
     } else {
-      unit_values<Real>(g, d, r, b, vals + pt * B * d.sv);
-      unit_vectors<Real>(g, d, r, b, vecs + pt * B * d.sc, ph);
+      unit_generic<Real, !Fast>(g, d, r, s_q + 3 * pl, b, ov, oc, ph_in_lds ? s_ph + NPH * pl : nullptr);
     }
   }
 }

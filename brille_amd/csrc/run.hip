@@ -74,12 +74,7 @@ int check_rotation_modes(const brille_grid* G) {
   return BRILLE_OK;
 }
 
-template <typename Real, bool Fast, int L>
-static void launch(const brille_grid* G, const LaunchArgs& a, hipStream_t s) {
-  const uint64_t blocks = (a.n + TILE - 1) / TILE;
-  if (!blocks) return;
-  hipLaunchKernelGGL((k_ir_interpolate<Real, Fast, L>), dim3((unsigned)blocks), dim3(TILE), 0, s, G->g, G->d, a);
-}
+using TileKernel = void (*)(GridConst, DataConst, LaunchArgs);
 
 // which specialised phase 2 the payload admits
 static int payload_layout(const DataConst& d) {
@@ -92,20 +87,41 @@ static int payload_layout(const DataConst& d) {
   return LAYOUT_GENERIC;
 }
 
+// The instantiated specialisations: Gamma payloads of 1-4 atoms and scalar
+// payloads of 1-4 complex elements, each with one value per mode; anything
+// else runs the general phase 2.
+template <typename Real>
+static TileKernel pick_tuned(const DataConst& d) {
+  const int L = payload_layout(d);
+  if (L == LAYOUT_GAMMA && d.sv == 1) {
+    switch (d.sc) {
+      case 3: return k_ir_interpolate<Real, true, LAYOUT_GAMMA, 3, 1>;
+      case 6: return k_ir_interpolate<Real, true, LAYOUT_GAMMA, 6, 1>;
+      case 9: return k_ir_interpolate<Real, true, LAYOUT_GAMMA, 9, 1>;
+      case 12: return k_ir_interpolate<Real, true, LAYOUT_GAMMA, 12, 1>;
+      default: break;
+    }
+  }
+  if (L == LAYOUT_SCALAR && d.sv == 1) {
+    switch (d.sc) {
+      case 1: return k_ir_interpolate<Real, true, LAYOUT_SCALAR, 1, 1>;
+      case 2: return k_ir_interpolate<Real, true, LAYOUT_SCALAR, 2, 1>;
+      case 3: return k_ir_interpolate<Real, true, LAYOUT_SCALAR, 3, 1>;
+      case 4: return k_ir_interpolate<Real, true, LAYOUT_SCALAR, 4, 1>;
+      default: break;
+    }
+  }
+  return k_ir_interpolate<Real, true, LAYOUT_GENERIC, 0, 0>;
+}
+
 // plain: the literal FP64 restatement; tuned: fast fold-in and specialised sums
 static void launch_sel(const brille_grid* G, const LaunchArgs& a, int precision, bool fast, hipStream_t s) {
-  const int L = payload_layout(G->d);
-  if (precision == BRILLE_FP32) {
-    if (!fast) launch<float, false, LAYOUT_GENERIC>(G, a, s);
-    else if (L == LAYOUT_GAMMA) launch<float, true, LAYOUT_GAMMA>(G, a, s);
-    else if (L == LAYOUT_SCALAR) launch<float, true, LAYOUT_SCALAR>(G, a, s);
-    else launch<float, true, LAYOUT_GENERIC>(G, a, s);
-  } else {
-    if (!fast) launch<double, false, LAYOUT_GENERIC>(G, a, s);
-    else if (L == LAYOUT_GAMMA) launch<double, true, LAYOUT_GAMMA>(G, a, s);
-    else if (L == LAYOUT_SCALAR) launch<double, true, LAYOUT_SCALAR>(G, a, s);
-    else launch<double, true, LAYOUT_GENERIC>(G, a, s);
-  }
+  const uint64_t blocks = (a.n + TILE - 1) / TILE;
+  if (!blocks) return;
+  TileKernel k;
+  if (precision == BRILLE_FP32) k = fast ? pick_tuned<float>(G->d) : k_ir_interpolate<float, false, LAYOUT_GENERIC, 0, 0>;
+  else k = fast ? pick_tuned<double>(G->d) : k_ir_interpolate<double, false, LAYOUT_GENERIC, 0, 0>;
+  hipLaunchKernelGGL(k, dim3((unsigned)blocks), dim3(TILE), 0, s, G->g, G->d, a);
 }
 
 int report_point_error(const ErrRec& e, const double* Q, uint64_t n, bool host_mem) {
