@@ -46,11 +46,24 @@ def main():
     ap.add_argument("--n-q", type=float, default=None)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline (profiling runs)")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only copy this many MiB device-to-device once (calibrates the HBM byte counters) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               "(the path shards by Q with no exchange; see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_probe_mib:
+        import torch
+        x = torch.ones(args.copy_probe_mib << 20, dtype=torch.uint8, device="cuda")
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        y.copy_(x)
+        torch.cuda.synchronize()
+        log(f"copied {args.copy_probe_mib} MiB device-to-device")
+        return
 
     from brille_amd import capi
     from brille_amd import synthetic as S
@@ -115,25 +128,11 @@ def main():
     tb = table_bytes(T)
     alg = n * bytes_per_q + tb
     achieved = alg / (k_fast * 1e-3) / 1e9
+    per_q = measured_traffic(cfg)
     roofline = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
-                    traffic=None, bytes_per_q=bytes_per_q, table_bytes=tb, kernel="k_ir_interpolate<float,true>")
-
-    # CPU baseline: the oracle (C, OpenMP, the reference's loop structure) on a bounded sample
-    from oracle import oracle as O
-    cores = min(16, os.cpu_count() or 1)
-    probe = S.q_points(cfg, 20000, stream=7)
-    tp = time.perf_counter()
-    O.ir_interpolate_at(T, probe, nthreads=cores)
-    rate = len(probe) / (time.perf_counter() - tp)
-    m = int(min(max(rate * args.cpu_seconds, 20000), 5e6))
-    sample = S.q_points(cfg, m, stream=8)
-    tc = time.perf_counter()
-    rc, *_ = O.ir_interpolate_at(T, sample, nthreads=cores)
-    cpu_s = time.perf_counter() - tc
-    cpu = dict(value=m / cpu_s, unit="Q-points/s", cores=cores, kind="port",
-               sample=f"{m} Q of the same workload (uniform in [-{c['box']},{c['box']}]^3), FP64, {cores} OpenMP threads")
-    log(f"cpu baseline: {m / cpu_s:.4g} Q/s on {cores} threads")
-
+                    traffic=(per_q * n if per_q else None), bytes_per_q=bytes_per_q, table_bytes=tb,
+                    traffic_bytes_per_q=per_q, kernel="k_ir_interpolate<float, tuned, Gamma layout>")
+    cpu = None if args.no_cpu else cpu_baseline(T, cfg, c, args.cpu_seconds)
     out = {
         "metric": "Q-points/s ir_interpolate_at, 1 MI355X: learned+batched vs plain, %HBM peak",
         "value": n / wall_fast * 1e3,
@@ -157,6 +156,38 @@ def main():
         "speedup_vs_plain": (plain_res["ms_per_step"] / wall_fast) if plain_res else None,
     }
     print(json.dumps(out), flush=True)
+
+
+def measured_traffic(cfg):
+    """HBM bytes per Q of the headline kernel, from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE per dispatch over Q per dispatch, scaled by
+    the copy-kernel calibration); None when no summary exists for this config."""
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_k_ir_interpolate_config{cfg}.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_q:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def cpu_baseline(T, cfg, c, seconds):
+    """The oracle (C, OpenMP, the reference's loop structure) on a bounded sample."""
+    from brille_amd import synthetic as S
+    from oracle import oracle as O
+    cores = min(16, os.cpu_count() or 1)
+    probe = S.q_points(cfg, 20000, stream=7)
+    tp = time.perf_counter()
+    O.ir_interpolate_at(T, probe, nthreads=cores)
+    rate = len(probe) / (time.perf_counter() - tp)
+    m = int(min(max(rate * seconds, 20000), 5e6))
+    sample = S.q_points(cfg, m, stream=8)
+    tc = time.perf_counter()
+    rc, *_ = O.ir_interpolate_at(T, sample, nthreads=cores)
+    cpu_s = time.perf_counter() - tc
+    log(f"cpu baseline: {m / cpu_s:.4g} Q/s on {cores} threads")
+    return dict(value=m / cpu_s, unit="Q-points/s", cores=cores, kind="port",
+                sample=f"{m} Q of the same workload (uniform in [-{c['box']},{c['box']}]^3), FP64, {cores} OpenMP threads")
 
 
 if __name__ == "__main__":
