@@ -72,6 +72,8 @@ def main():
     if lib.brille_device_count() < 1:
         raise SystemExit("no HIP device")
     cfg = args.config
+    if cfg == 2:
+        return bench_batch(args, capi, S)
     c = S.CONFIGS[cfg]
     n = int(args.n_q if args.n_q is not None else c["n_q"])
     t0 = time.time()
@@ -156,6 +158,103 @@ def main():
         "speedup_vs_plain": (plain_res["ms_per_step"] / wall_fast) if plain_res else None,
     }
     print(json.dumps(out), flush=True)
+
+
+def bench_batch(args, capi, S):
+    """Configuration 2: 64 uneven grids, 1e7 Q, one persistent graph launch
+    with the autotuned schedule, against one tile-kernel call per grid."""
+    import ctypes as C
+    lib = capi.lib()
+    n_q = int(args.n_q) if args.n_q is not None else 10_000_000
+    t0 = time.time()
+    grids, Q, offsets = S.batch_grids(n_grids=64, n_q=n_q)
+    tables = [g.tables() for g in grids]
+    devs = [capi.Grid(t) for t in tables]
+    V = [int(t["vertices"].shape[0]) for t in tables]
+    Bm = [int(t["branches"]) for t in tables]
+    log(f"64 grids built in {time.time() - t0:.1f}s: V {min(V)}..{max(V)} (sum {sum(V)}), modes {min(Bm)}..{max(Bm)}")
+    batch = capi.Batch(devs)
+    dQ = capi.DeviceArray.from_host(Q)
+    vals, vecs = batch.outputs(offsets, capi.FP32)
+    capi.check(batch.run(dQ, offsets, vals, vecs, precision=capi.FP32, autotune=True))
+    sch = batch.schedule()
+    log(f"autotuned schedule: {sch}")
+    for _ in range(args.warmup):
+        batch.time(dQ, offsets, vals, vecs, precision=capi.FP32, iters=1)
+    w0 = time.perf_counter()
+    kms = [batch.time(dQ, offsets, vals, vecs, precision=capi.FP32, iters=1) for _ in range(args.steps)]
+    capi.check(lib.brille_device_sync())
+    wall = (time.perf_counter() - w0) * 1000.0 / args.steps
+    seq = batch.time(dQ, offsets, vals, vecs, precision=capi.FP32, kernel=capi.KERNEL_TILE, iters=max(1, args.steps))
+    plain = None
+    if not args.no_plain:
+        v64, c64 = batch.outputs(offsets, capi.FP64)
+        plain = batch.time(dQ, offsets, v64, c64, precision=capi.FP64, kernel=capi.KERNEL_PLAIN, iters=1)
+        del v64, c64
+    log(f"batched fp32: {wall:.3f} ms/step wall, graph {np.mean(kms):.3f} ms; per-grid tile calls {seq:.3f} ms; "
+        f"per-grid plain fp64 {plain if plain is None else round(plain, 3)} ms")
+    per_q = np.zeros(len(devs))
+    table_total = 0
+    for g, t in enumerate(tables):
+        sv = int(sum(t["values_elements"]))
+        sc = int(sum(t["vectors_elements"]))
+        per_q[g] = 24 + Bm[g] * sv * 4 + Bm[g] * sc * 8
+        table_total += table_bytes(t)
+    counts = np.diff(offsets.astype(np.int64))
+    alg = float((counts * per_q).sum()) + table_total
+    achieved = alg / (np.mean(kms) * 1e-3) / 1e9
+    cpu = None
+    if not args.no_cpu:
+        cpu = batch_cpu_baseline(tables, Q, offsets, args.cpu_seconds)
+    out = {
+        "metric": "Q-points/s ir_interpolate_at, 1 MI355X: learned+batched vs plain, %HBM peak",
+        "value": n_q / wall * 1e3,
+        "unit": "Q-points/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"config2: 64 grids (space groups 221/139/62/14/194 cycling), V {min(V)}..{max(V)}, "
+                               f"3..72 modes, {n_q:.3g} Q split Dirichlet(0.3), one persistent graph launch",
+                   "n_q": n_q, "vertices_total": int(sum(V)), "schedule": sch, "parallelism": "single GPU"},
+        "roofline": dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
+                         traffic=None, kernel="k_batch<float>"),
+        "cpu_baseline": cpu,
+        "per_grid_tile_ms": seq,
+        "per_grid_plain_fp64_ms": plain,
+        "speedup_vs_plain": (plain / wall) if plain else None,
+        "speedup_vs_per_grid_calls": seq / np.mean(kms),
+    }
+    print(json.dumps(out), flush=True)
+
+
+def batch_cpu_baseline(tables, Q, offsets, seconds):
+    """The oracle on a bounded sample: the same fraction of every grid's points."""
+    from oracle import oracle as O
+    cores = min(16, os.cpu_count() or 1)
+    counts = np.diff(offsets.astype(np.int64))
+    frac = 2e-3
+    total, tsum = 0, 0.0
+    for g, t in enumerate(tables):
+        m = int(counts[g] * frac)
+        if m == 0:
+            continue
+        a = int(offsets[g])
+        tc = time.perf_counter()
+        O.ir_interpolate_at(t, Q[a:a + m], nthreads=cores)
+        tsum += time.perf_counter() - tc
+        total += m
+        if tsum > seconds:
+            break
+    log(f"cpu baseline: {total / tsum:.4g} Q/s on {cores} threads ({total} Q)")
+    return dict(value=total / tsum, unit="Q-points/s", cores=cores, kind="port",
+                sample=f"{total} Q: the first {frac:g} of each grid's points (grids in order until {seconds:g} s), "
+                       f"FP64, {cores} OpenMP threads")
 
 
 def measured_traffic(cfg):

@@ -19,7 +19,7 @@ STATUS_NAMES = {
 }
 FP64, FP32 = 0, 1
 MEM_DEVICE, MEM_HOST = 0, 1
-KERNEL_AUTO, KERNEL_PLAIN, KERNEL_TUNED = 0, 1, 2
+KERNEL_AUTO, KERNEL_PLAIN, KERNEL_TUNED, KERNEL_TILE, KERNEL_BINNED = 0, 1, 2, 3, 4
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
@@ -112,7 +112,7 @@ SIGNATURES = {
                                    C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_void_p]),
     "brille_batch_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(RunOpts),
                                         C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_void_p]),
-    "brille_batch_schedule": (C.c_int, [C.c_void_p, _up, _up, _up]),
+    "brille_batch_schedule": (C.c_int, [C.c_void_p, _up, _up, _up, _up]),
     "brille_batch_destroy": (C.c_int, [C.c_void_p]),
     "brille_proposal_train": (C.c_int, [C.c_void_p, C.POINTER(TrainOpts), C.POINTER(TrainReport)]),
     "brille_proposal_stats": (C.c_int, [C.c_void_p, _u64p, _u64p]),
@@ -120,6 +120,7 @@ SIGNATURES = {
                                             C.POINTER(C.c_float), C.POINTER(C.c_float), _up]),
     "brille_proposal_get_params": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), _up]),
     "brille_proposal_set_params": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_uint32]),
+    "brille_proposal_predict": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), _ip, C.c_uint32, _ip]),
     "brille_time_ir_interpolate_at": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(RunOpts), C.c_void_p,
                                                 C.c_void_p, C.c_uint32, C.c_void_p, C.POINTER(KernelTimes)]),
     "brille_time_batch_run": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(RunOpts),
@@ -397,6 +398,63 @@ class Grid:
         try:
             if getattr(self, "h", None):
                 lib().brille_grid_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+class Batch:
+    """Many grid handles run in one persistent launch (brille_batch_*), on
+    device buffers: Q of all grids back to back, one output pair per grid."""
+
+    def __init__(self, grids):
+        self.grids = list(grids)  # the handles must outlive the batch
+        arr = (C.c_void_p * len(self.grids))(*[g.h for g in self.grids])
+        h = C.c_void_p()
+        check(lib().brille_batch_create(arr, len(self.grids), C.byref(h)))
+        self.h = h
+
+    def outputs(self, offsets, precision):
+        """Device output buffers for every grid."""
+        vd = np.float32 if precision == FP32 else np.float64
+        cd = np.complex64 if precision == FP32 else np.complex128
+        vals, vecs = [], []
+        for g, grid in enumerate(self.grids):
+            n = int(offsets[g + 1] - offsets[g])
+            vs, cs = grid.out_shapes(n)
+            vals.append(DeviceArray(vs, vd))
+            vecs.append(DeviceArray(cs, cd))
+        return vals, vecs
+
+    def _args(self, offsets, vals, vecs):
+        off = np.ascontiguousarray(offsets, dtype=np.uint64)
+        vp = (C.c_void_p * len(vals))(*[v.ptr for v in vals])
+        cp = (C.c_void_p * len(vecs))(*[c.ptr for c in vecs])
+        return off, vp, cp
+
+    def run(self, dQ, offsets, vals, vecs, precision=FP64, kernel=KERNEL_AUTO, check_errors=True, autotune=False):
+        off, vp, cp = self._args(offsets, vals, vecs)
+        o = RunOpts(0, precision, 0, MEM_DEVICE, kernel, int(check_errors))
+        f = lib().brille_batch_autotune if autotune else lib().brille_batch_run
+        return f(self.h, dQ.ptr, off.ctypes.data_as(_u64p), C.byref(o), vp, cp, None)
+
+    def time(self, dQ, offsets, vals, vecs, precision=FP32, kernel=KERNEL_AUTO, iters=3):
+        off, vp, cp = self._args(offsets, vals, vecs)
+        o = RunOpts(0, precision, 0, MEM_DEVICE, kernel, 0)
+        t = KernelTimes()
+        check(lib().brille_time_batch_run(self.h, dQ.ptr, off.ctypes.data_as(_u64p), C.byref(o), vp, cp, iters,
+                                          None, C.byref(t)))
+        return t.total_ms
+
+    def schedule(self):
+        v = [C.c_uint32() for _ in range(4)]
+        check(lib().brille_batch_schedule(self.h, *[C.byref(x) for x in v]))
+        return dict(tile_points=v[0].value, blocks=v[1].value, tiles_per_grab=v[2].value, order=v[3].value)
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                lib().brille_batch_destroy(self.h)
                 self.h = None
         except Exception:
             pass

@@ -1,0 +1,406 @@
+// Binned ir_interpolate_at for large calls.
+//
+// With Q spread over the whole zone, the eight payload rows a point gathers
+// are unrelated to those of its neighbours in the input, and on a payload
+// larger than the L2 nearly every row comes from HBM.  This pipeline first
+// orders the points by trellis node range, so that the points one tile sums
+// share their vertices and the rows hit in cache:
+//   k_bin_fold     stage (a) + node subscript, one lane per point; a 32-byte
+//                  FoldRec per point in input order and a per-block histogram
+//                  of node-range buckets (failed points get the last bucket);
+//   k_scan_*       exclusive scan of the block histograms (bucket-major);
+//   k_bin_scatter  each block moves its FoldRecs to its reserved slots of each
+//                  bucket (a SortRec adds the input index);
+//   k_bin_sums     the rest of stage (b) and stages (c1, c2) on bucket-ordered
+//                  tiles, writing output rows at the input index.
+// Within a bucket the order is whatever the LDS atomics give; every point is
+// computed independently, so the outputs do not depend on it.
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+
+#include "grid_state.hpp"
+
+namespace brille_amd {
+
+constexpr int BIN_BLOCK = 256;
+constexpr int BIN_PPT = 8;                            // points per thread in the bucketing kernels
+constexpr uint32_t BIN_SPAN = BIN_BLOCK * BIN_PPT;   // points per bucketing block
+constexpr uint64_t BIN_CHUNK = 1ull << 24;           // points per pass of the pipeline
+constexpr uint32_t SCAN_SEG = 4096;                   // elements per scan block (256 x 16)
+
+struct FoldRec {
+  double q[3];
+  uint32_t node;
+  uint32_t meta;  // rot | invrot << 8 | status << 16
+};
+struct SortRec {
+  double q[3];
+  uint32_t node;
+  uint32_t meta;
+  uint32_t index;  // input index within the pass
+  uint32_t hint;   // proposed tetrahedron (kNoTet: none)
+};
+
+__device__ __forceinline__ uint32_t bucket_of(uint32_t node, uint32_t meta, uint32_t nb, uint64_t n_nodes) {
+  return (meta >> 16) != BRILLE_PT_OK ? nb : (uint32_t)(((uint64_t)node * nb) / n_nodes);
+}
+
+template <bool Fast>
+__global__ __launch_bounds__(BIN_BLOCK) void k_bin_fold(GridConst g, const double* Q, uint32_t n, int no_move,
+                                                        FoldRec* rec, uint32_t* hist, uint32_t nblk, uint32_t nb,
+                                                        ErrRec* err, uint64_t base) {
+  extern __shared__ uint32_t h[];  // nb + 1 counters
+  for (uint32_t k = threadIdx.x; k <= nb; k += BIN_BLOCK) h[k] = 0;
+  __syncthreads();
+  const uint32_t b0 = blockIdx.x * BIN_SPAN;
+  for (int i = 0; i < BIN_PPT; ++i) {
+    const uint32_t p = b0 + i * BIN_BLOCK + threadIdx.x;
+    if (p < n) {
+      double Qp[3] = {Q[3 * (uint64_t)p], Q[3 * (uint64_t)p + 1], Q[3 * (uint64_t)p + 2]};
+      Located L;
+      fold_node<Fast>(g, Qp, no_move, L);
+      if (L.status != BRILLE_PT_OK) report_point(err, base + p, L.status);
+      FoldRec r;
+      r.q[0] = L.q[0]; r.q[1] = L.q[1]; r.q[2] = L.q[2];
+      r.node = L.node;
+      r.meta = (L.rot & 255u) | ((L.invrot & 255u) << 8) | ((uint32_t)L.status << 16);
+      rec[p] = r;
+      atomicAdd(&h[bucket_of(r.node, r.meta, nb, g.n_nodes)], 1u);
+    }
+  }
+  __syncthreads();
+  for (uint32_t k = threadIdx.x; k <= nb; k += BIN_BLOCK) hist[(uint64_t)k * nblk + blockIdx.x] = h[k];
+}
+
+// exclusive scan of 256 values held one per thread; returns the thread's
+// exclusive prefix, *total gets the sum
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* total) {
+  __shared__ uint32_t wsum[BIN_BLOCK / 64];
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  uint32_t x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    uint32_t y = __shfl_up(x, o, 64);
+    if (lane >= (uint32_t)o) x += y;
+  }
+  if (lane == 63) wsum[wave] = x;
+  __syncthreads();
+  uint32_t before = 0, all = 0;
+#pragma unroll
+  for (int w = 0; w < BIN_BLOCK / 64; ++w) {
+    if ((uint32_t)w < wave) before += wsum[w];
+    all += wsum[w];
+  }
+  __syncthreads();
+  *total = all;
+  return before + x - v;
+}
+
+__global__ __launch_bounds__(BIN_BLOCK) void k_scan_reduce(const uint32_t* a, uint64_t m, uint32_t* part) {
+  const uint64_t s0 = (uint64_t)blockIdx.x * SCAN_SEG;
+  uint32_t v = 0;
+  for (uint32_t j = 0; j < SCAN_SEG / BIN_BLOCK; ++j) {
+    const uint64_t i = s0 + (uint64_t)threadIdx.x * (SCAN_SEG / BIN_BLOCK) + j;
+    if (i < m) v += a[i];
+  }
+  uint32_t total;
+  (void)block_exclusive_scan(v, &total);
+  if (threadIdx.x == 0) part[blockIdx.x] = total;
+}
+
+// one block: exclusive scan of the segment sums in place
+__global__ __launch_bounds__(BIN_BLOCK) void k_scan_top(uint32_t* part, uint32_t np) {
+  const uint32_t per = (np + BIN_BLOCK - 1) / BIN_BLOCK;
+  const uint32_t i0 = threadIdx.x * per;
+  uint32_t v = 0;
+  for (uint32_t j = 0; j < per; ++j)
+    if (i0 + j < np) v += part[i0 + j];
+  uint32_t total;
+  uint32_t run = block_exclusive_scan(v, &total);
+  for (uint32_t j = 0; j < per; ++j)
+    if (i0 + j < np) {
+      const uint32_t t = part[i0 + j];
+      part[i0 + j] = run;
+      run += t;
+    }
+}
+
+__global__ __launch_bounds__(BIN_BLOCK) void k_scan_apply(uint32_t* a, uint64_t m, const uint32_t* part) {
+  constexpr uint32_t PER = SCAN_SEG / BIN_BLOCK;
+  const uint64_t i0 = (uint64_t)blockIdx.x * SCAN_SEG + (uint64_t)threadIdx.x * PER;
+  uint32_t v[PER];
+  uint32_t s = 0;
+#pragma unroll
+  for (uint32_t j = 0; j < PER; ++j) {
+    v[j] = i0 + j < m ? a[i0 + j] : 0u;
+    s += v[j];
+  }
+  uint32_t total;
+  uint32_t run = block_exclusive_scan(s, &total) + part[blockIdx.x];
+#pragma unroll
+  for (uint32_t j = 0; j < PER; ++j) {
+    if (i0 + j < m) a[i0 + j] = run;
+    run += v[j];
+  }
+}
+
+__global__ __launch_bounds__(BIN_BLOCK) void k_bin_scatter(const FoldRec* rec, uint32_t n, const uint32_t* off,
+                                                           uint32_t nblk, uint32_t nb, uint64_t n_nodes, SortRec* out) {
+  extern __shared__ uint32_t s[];  // base[nb + 1], count[nb + 1]
+  uint32_t* base = s;
+  uint32_t* cnt = s + nb + 1;
+  for (uint32_t k = threadIdx.x; k <= nb; k += BIN_BLOCK) {
+    base[k] = off[(uint64_t)k * nblk + blockIdx.x];
+    cnt[k] = 0;
+  }
+  __syncthreads();
+  const uint32_t b0 = blockIdx.x * BIN_SPAN;
+  for (int i = 0; i < BIN_PPT; ++i) {
+    const uint32_t p = b0 + i * BIN_BLOCK + threadIdx.x;
+    if (p < n) {
+      const FoldRec r = rec[p];
+      const uint32_t k = bucket_of(r.node, r.meta, nb, n_nodes);
+      const uint32_t j = atomicAdd(&cnt[k], 1u);
+      SortRec o;
+      o.q[0] = r.q[0]; o.q[1] = r.q[1]; o.q[2] = r.q[2];
+      o.node = r.node;
+      o.meta = r.meta;
+      o.index = p;
+      o.hint = kNoTet;
+      out[base[k] + j] = o;
+    }
+  }
+}
+
+// the learned proposal over bucket-ordered records: the points in polyhedron
+// nodes are compacted per tile and the network's choice is stored in the
+// record for k_bin_sums to verify
+__global__ __launch_bounds__(TILE) void k_bin_propose(GridConst g, ProposalConst pc, SortRec* srt, uint32_t n) {
+  __shared__ PropQueue<TILE> s_pq;
+  const uint32_t p = blockIdx.x * TILE + threadIdx.x;
+  bool want = false;
+  double q[3] = {0.0, 0.0, 0.0};
+  uint32_t node = 0;
+  if (p < n) {
+    const SortRec& sr = srt[p];
+    node = sr.node;
+    want = (sr.meta >> 16) == BRILLE_PT_OK && g.node_type[node] == BRILLE_NODE_POLY;
+    q[0] = sr.q[0]; q[1] = sr.q[1]; q[2] = sr.q[2];
+  }
+  const uint32_t hint = propose_block<TILE>(g, pc, s_pq, want, q, node);
+  if (want) srt[p].hint = hint;
+}
+
+template <typename Real, int L, int SC, int SV, bool Prop>
+__global__ __launch_bounds__(TILE) void k_bin_sums(GridConst g, DataConst d, ProposalConst pc, const SortRec* srt,
+                                                    uint32_t n, void* vals, void* vecs, brille_aux_out aux,
+                                                    ErrRec* err, uint64_t base) {
+  using C = typename Cplx<Real>::T;
+  using S = TileShape<L, SC>;
+  __shared__ PointRec<Real> s_rec[TILE];
+  __shared__ C s_ph[TILE * S::NPH];
+  __shared__ double s_q[S::NQ];
+  __shared__ uint32_t s_out[TILE];
+  __shared__ uint32_t s_tried, s_verified;
+  if (Prop && threadIdx.x == 0) {
+    s_tried = 0;
+    s_verified = 0;
+  }
+  const uint32_t tile0 = blockIdx.x * TILE;
+  const uint32_t tid = threadIdx.x;
+  const uint32_t p = tile0 + tid;
+  const bool want_phase = d.vec_rot == BRILLE_ROT_GAMMA && d.ec[1] > 0;
+  const uint32_t n_atoms = d.ec[1] / 3u;
+  const bool ph_in_lds = L == LAYOUT_GAMMA || (want_phase && n_atoms <= 4);
+  if (Prop) __syncthreads();
+  {
+    PointRec<Real>& r = s_rec[tid];
+    r.nslot = 0;
+    if (p < n) {
+      const SortRec sr = srt[p];
+      s_out[tid] = sr.index;
+      Located P;
+      P.q[0] = sr.q[0]; P.q[1] = sr.q[1]; P.q[2] = sr.q[2];
+      P.tau[0] = P.tau[1] = P.tau[2] = 0;
+      P.node = sr.node;
+      P.rot = sr.meta & 255u;
+      P.invrot = (sr.meta >> 8) & 255u;
+      P.status = (int)(sr.meta >> 16);
+      P.tet = kNoTet;
+      P.nslot = 0;
+      P.nkept = 0;
+      const bool folded = P.status == BRILLE_PT_OK;
+      if (folded) {
+        if constexpr (Prop) {
+          bool verified = false;
+          node_slots(g, P, sr.hint, pc.margin, &verified);
+          if (sr.hint != kNoTet) {
+            atomicAdd(&s_tried, 1u);
+            if (verified) atomicAdd(&s_verified, 1u);
+          }
+        } else {
+          node_slots(g, P);
+        }
+        // failed fold-ins were reported by k_bin_fold
+        if (P.status != BRILLE_PT_OK) report_point(err, base + sr.index, P.status);
+        else store_record<Real, L, SC>(d, P, r, s_ph, s_q, tid, ph_in_lds, n_atoms);
+      }
+      write_aux(aux, sr.index, P);
+    }
+  }
+  __syncthreads();
+  if (Prop && threadIdx.x == 0 && s_tried) {
+    atomicAdd(&pc.stats[0], (unsigned long long)s_tried);
+    atomicAdd(&pc.stats[1], (unsigned long long)s_verified);
+  }
+  const uint32_t npts = n - tile0 < (uint32_t)TILE ? n - tile0 : (uint32_t)TILE;
+  tile_sums<Real, true, L, SC, SV>(g, d, s_rec, s_ph, s_q, s_out, 0, npts, ph_in_lds, vals, vecs);
+}
+
+using SumsKernel = void (*)(GridConst, DataConst, ProposalConst, const SortRec*, uint32_t, void*, void*,
+                           brille_aux_out, ErrRec*, uint64_t);
+
+template <typename Real>
+static SumsKernel pick_sums(const DataConst& d, bool prop) {
+  const int L = payload_layout(d);
+  if (prop) {
+    // the learned proposal is instantiated for the two-atom Gamma payload and the general layout
+    if (L == LAYOUT_GAMMA && d.sv == 1 && d.sc == 6) return k_bin_sums<Real, LAYOUT_GAMMA, 6, 1, true>;
+    return k_bin_sums<Real, LAYOUT_GENERIC, 0, 0, true>;
+  }
+  if (L == LAYOUT_GAMMA && d.sv == 1) {
+    switch (d.sc) {
+      case 3: return k_bin_sums<Real, LAYOUT_GAMMA, 3, 1, false>;
+      case 6: return k_bin_sums<Real, LAYOUT_GAMMA, 6, 1, false>;
+      case 9: return k_bin_sums<Real, LAYOUT_GAMMA, 9, 1, false>;
+      case 12: return k_bin_sums<Real, LAYOUT_GAMMA, 12, 1, false>;
+      default: break;
+    }
+  }
+  if (L == LAYOUT_SCALAR && d.sv == 1) {
+    switch (d.sc) {
+      case 1: return k_bin_sums<Real, LAYOUT_SCALAR, 1, 1, false>;
+      case 2: return k_bin_sums<Real, LAYOUT_SCALAR, 2, 1, false>;
+      case 3: return k_bin_sums<Real, LAYOUT_SCALAR, 3, 1, false>;
+      case 4: return k_bin_sums<Real, LAYOUT_SCALAR, 4, 1, false>;
+      default: break;
+    }
+  }
+  return k_bin_sums<Real, LAYOUT_GENERIC, 0, 0, false>;
+}
+
+// the aux outputs of a pass that starts at point c0
+static brille_aux_out offset_aux(const brille_aux_out& x, uint64_t c0) {
+  brille_aux_out o = x;
+  if (o.ir_q) o.ir_q += 3 * c0;
+  if (o.tau) o.tau += 3 * c0;
+  if (o.rot) o.rot += c0;
+  if (o.invrot) o.invrot += c0;
+  if (o.node) o.node += c0;
+  if (o.tet) o.tet += c0;
+  if (o.n_iw) o.n_iw += c0;
+  if (o.iw_index) o.iw_index += 8 * c0;
+  if (o.iw_weight) o.iw_weight += 8 * c0;
+  if (o.status) o.status += c0;
+  return o;
+}
+
+// buckets: enough that one bucket's share of the payload is about 64 KiB
+static uint32_t bucket_count(const brille_grid* G, int precision) {
+  const DataConst& d = G->d;
+  const uint64_t row = (uint64_t)(precision == BRILLE_FP32 ? d.stride32 * 4 : d.stride64 * 8);
+  const uint64_t payload = (uint64_t)d.V * d.B * row;
+  uint32_t nb = 16;
+  while (nb < 2048 && (uint64_t)nb * 65536 < payload) nb <<= 1;
+  return (uint32_t)std::min<uint64_t>(nb, std::max<uint64_t>(G->g.n_nodes, 1));
+}
+
+bool binned_applies(const brille_grid* G, const LaunchArgs& a, int kernel) {
+  if (kernel == BRILLE_KERNEL_PLAIN || kernel == BRILLE_KERNEL_TILE) return false;
+  const brille_aux_out& x = a.aux;
+  if (x.tau || G->g.G > 255) return false;  // the records carry no tau; Ridx/invRidx are bytes
+  if (a.use_proposal || kernel == BRILLE_KERNEL_BINNED) return true;
+  if (x.ir_q || x.rot || x.invrot || x.node || x.tet || x.n_iw || x.iw_index || x.iw_weight || x.status)
+    return false;
+  // automatic: large calls on payloads that do not fit the L2 of one XCD
+  const DataConst& d = G->d;
+  const uint64_t payload = (uint64_t)d.V * d.B * d.stride32 * 4;
+  return a.n >= (1ull << 18) && payload > (8ull << 20);
+}
+
+void free_binned(brille_grid* G) {
+  if (G->bin_ws) (void)hipFree(G->bin_ws);
+  G->bin_ws = nullptr;
+  G->bin_ws_bytes = 0;
+}
+
+// stage_ms (nullable): adds the milliseconds of fold, bucket (scan + scatter)
+// and sums over all passes; timing synchronises after every pass
+int launch_binned(brille_grid* G, const LaunchArgs& a, int precision, hipStream_t s, float* stage_ms) {
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  if (stage_ms)
+    for (auto& e : ev)
+      if (hipEventCreate(&e) != hipSuccess) return fail(BRILLE_ERR_DEVICE, "hipEventCreate failed");
+  hipEvent_t* evp = stage_ms ? ev : nullptr;
+  const uint64_t chunk = std::min<uint64_t>(a.n, BIN_CHUNK);
+  const uint32_t nb = bucket_count(G, precision);
+  const uint32_t nblk_max = (uint32_t)((chunk + BIN_SPAN - 1) / BIN_SPAN);
+  const uint64_t m_max = (uint64_t)(nb + 1) * nblk_max;
+  const uint64_t np_max = (m_max + SCAN_SEG - 1) / SCAN_SEG;
+  auto al = [](uint64_t b) { return (b + 255) & ~uint64_t(255); };
+  const uint64_t o_rec = 0, o_srt = al(chunk * sizeof(FoldRec)), o_hist = o_srt + al(chunk * sizeof(SortRec)),
+                 o_part = o_hist + al(m_max * 4), need = o_part + al(np_max * 4);
+  if (np_max > 256ull * 1024) return fail(BRILLE_ERR_INVALID_ARGUMENT, "binned pass too large");
+  if (G->bin_ws_bytes < need) {
+    free_binned(G);
+    if (hipMalloc(&G->bin_ws, need) != hipSuccess) return fail(BRILLE_ERR_DEVICE, "hipMalloc failed for the binned work space");
+    G->bin_ws_bytes = need;
+  }
+  char* ws = static_cast<char*>(G->bin_ws);
+  FoldRec* rec = reinterpret_cast<FoldRec*>(ws + o_rec);
+  SortRec* srt = reinterpret_cast<SortRec*>(ws + o_srt);
+  uint32_t* hist = reinterpret_cast<uint32_t*>(ws + o_hist);
+  uint32_t* part = reinterpret_cast<uint32_t*>(ws + o_part);
+  const bool prop = a.use_proposal != 0;
+  const SumsKernel sums = precision == BRILLE_FP32 ? pick_sums<float>(G->d, prop) : pick_sums<double>(G->d, prop);
+  const size_t rs = precision == BRILLE_FP32 ? 4 : 8;
+  const size_t vb = (size_t)G->d.B * G->d.sv * rs, cb = (size_t)G->d.B * G->d.sc * 2 * rs;
+  for (uint64_t c0 = 0; c0 < a.n; c0 += chunk) {
+    const uint32_t m = (uint32_t)std::min<uint64_t>(chunk, a.n - c0);
+    const uint32_t nblk = (m + BIN_SPAN - 1) / BIN_SPAN;
+    const uint64_t M = (uint64_t)(nb + 1) * nblk;
+    const uint32_t np = (uint32_t)((M + SCAN_SEG - 1) / SCAN_SEG);
+    const size_t lds1 = (size_t)(nb + 1) * 4;
+    if (evp) (void)hipEventRecord(evp[0], s);
+    hipLaunchKernelGGL(k_bin_fold<true>, dim3(nblk), dim3(BIN_BLOCK), lds1, s, G->g, a.Q + 3 * c0, m, a.no_move, rec,
+                       hist, nblk, nb, a.err, a.base + c0);
+    if (evp) (void)hipEventRecord(evp[1], s);
+    hipLaunchKernelGGL(k_scan_reduce, dim3(np), dim3(BIN_BLOCK), 0, s, hist, M, part);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(BIN_BLOCK), 0, s, part, np);
+    hipLaunchKernelGGL(k_scan_apply, dim3(np), dim3(BIN_BLOCK), 0, s, hist, M, part);
+    hipLaunchKernelGGL(k_bin_scatter, dim3(nblk), dim3(BIN_BLOCK), 2 * lds1, s, rec, m, hist, nblk, nb,
+                       G->g.n_nodes, srt);
+    if (evp) (void)hipEventRecord(evp[2], s);
+    if (prop)
+      hipLaunchKernelGGL(k_bin_propose, dim3((m + TILE - 1) / TILE), dim3(TILE), 0, s, G->g, a.prop, srt, m);
+    hipLaunchKernelGGL(sums, dim3((m + TILE - 1) / TILE), dim3(TILE), 0, s, G->g, G->d, a.prop, srt, m,
+                       static_cast<char*>(a.vals) + c0 * vb, static_cast<char*>(a.vecs) + c0 * cb,
+                       offset_aux(a.aux, c0), a.err, a.base + c0);
+    if (evp) {
+      (void)hipEventRecord(evp[3], s);
+      (void)hipEventSynchronize(evp[3]);
+      for (int k = 0; k < 3; ++k) {
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, evp[k], evp[k + 1]);
+        stage_ms[k] += ms;
+      }
+    }
+  }
+  if (evp)
+    for (auto& e : ev) (void)hipEventDestroy(e);
+  if (hipGetLastError() != hipSuccess) return fail(BRILLE_ERR_DEVICE, "binned launch failed");
+  return BRILLE_OK;
+}
+
+}  // namespace brille_amd

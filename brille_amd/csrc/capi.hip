@@ -24,6 +24,8 @@ int fail(int code, const std::string& msg) {
   return code;
 }
 
+const char* g_last_error_text() { return g_last_error.c_str(); }
+
 #define HIP_TRY(expr)                                                                              \
   do {                                                                                             \
     hipError_t e_ = (expr);                                                                        \
@@ -278,6 +280,7 @@ int brille_grid_destroy(brille_grid* G) {
   free_all(G->data_allocs);
   free_proposal(G);
   free_staging(G);
+  free_binned(G);
   if (G->err) (void)hipFree(G->err);
   delete G;
   return BRILLE_OK;

@@ -101,18 +101,24 @@ struct DataConst {
   const uint32_t* perms;
 };
 
-// Learned tetrahedron proposal: an MLP 6 -> H -> H -> K over the slots of a
-// polyhedron node, bf16 weights with FP32 accumulation.
+// Learned tetrahedron proposal: an MLP 3 -> 128 -> 128 -> K over the
+// tetrahedra of the trellis (K = number of tetrahedra), evaluated only for the
+// tetrahedra of the point's polyhedron node.  Inference weights are bf16
+// (FP32 accumulation); W1 and the biases stay FP32.
+constexpr int kPropH = 128;
 struct ProposalConst {
   int32_t enabled;
-  uint32_t K, H, pad;
-  const uint16_t* W1;  // H x 8 (6 used), bf16
+  uint32_t K;
+  float lo[3];     // features: (xyz - lo) * scale - 1, in [-1, 1] over the trellis box
+  float scale[3];
+  const float* W1;     // H x 3
   const float* b1;     // H
-  const uint16_t* W2;  // H x H
+  const uint16_t* W2;  // H x H bf16, row = output unit
   const float* b2;     // H
-  const uint16_t* W3;  // K x H
+  const uint16_t* W3;  // K x H bf16, row c stored lane-major: element nb*16+m at m*8+nb
   const float* b3;     // K
-  unsigned long long* stats;  // tried, hits
+  double margin;       // a proposal is taken when every barycentric weight exceeds it
+  unsigned long long* stats;  // tried, verified
 };
 
 struct Tol {

@@ -7,6 +7,7 @@
 
 #include "../../include/brille_amd.h"
 #include "interp_kernels.hpp"
+#include "proposal.hpp"
 
 struct brille_grid {
   brille_amd::GridConst g{};
@@ -20,20 +21,37 @@ struct brille_grid {
   // staging buffers for host-memory calls
   void* stage = nullptr;
   size_t stage_bytes = 0;
-  // learned tetrahedron proposal
+  // work space of the binned pipeline (grown on demand, kept)
+  void* bin_ws = nullptr;
+  size_t bin_ws_bytes = 0;
+  // learned tetrahedron proposal: FP32 master parameters (flat: W1 b1 W2 b2
+  // W3 b3), Adam moments and gradients; bf16 inference copies in `prop`
   brille_amd::ProposalConst prop{};
   std::vector<void*> prop_allocs;
   bool prop_trained = false;
-  unsigned long long* prop_stats = nullptr;  // device: tried, hits
+  uint32_t prop_K = 0;
+  uint64_t prop_n_params = 0;
+  float* prop_params = nullptr;
+  float* prop_m = nullptr;
+  float* prop_v = nullptr;
+  float* prop_grads = nullptr;
+  uint64_t prop_step = 0;
+  unsigned long long* prop_stats = nullptr;  // device: tried, verified
   unsigned long long last_tried = 0, last_hits = 0;
 };
 
 namespace brille_amd {
 int fail(int code, const std::string& msg);
+const char* g_last_error_text();  // the message of the last failure on this thread
 int build_fast_tables(const brille_grid_desc* desc, brille_grid* G);
 void free_proposal(brille_grid* G);
 void free_staging(brille_grid* G);
 int check_rotation_modes(const brille_grid* G);
+int payload_layout(const DataConst& d);
+// binned pipeline (binned.hip)
+bool binned_applies(const brille_grid* G, const LaunchArgs& a, int kernel);
+int launch_binned(brille_grid* G, const LaunchArgs& a, int precision, hipStream_t s, float* stage_ms);
+void free_binned(brille_grid* G);
 // the first failing point of a call as the reference's error text
 int report_point_error(const ErrRec& e, const double* Q, uint64_t n, bool host_mem);
 int run_interpolate(brille_grid* G, const double* Q, uint64_t n, const brille_run_opts* opts, void* values_out,

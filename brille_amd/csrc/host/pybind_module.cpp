@@ -109,6 +109,8 @@ py::tuple grid_ir_interpolate(PyGrid& pg, const darr& Q, bool useparallel, int t
   int kern = BRILLE_KERNEL_AUTO;
   if (kernel == "plain") kern = BRILLE_KERNEL_PLAIN;
   else if (kernel == "tuned") kern = BRILLE_KERNEL_TUNED;
+  else if (kernel == "tile") kern = BRILLE_KERNEL_TILE;
+  else if (kernel == "binned") kern = BRILLE_KERNEL_BINNED;
   const size_t n = static_cast<size_t>(Q.shape(0));
   std::vector<py::ssize_t> vs{static_cast<py::ssize_t>(n)}, cs{static_cast<py::ssize_t>(n)};
   for (size_t i = 1; i < g.values.shape.size(); ++i) vs.push_back(static_cast<py::ssize_t>(g.values.shape[i]));

@@ -123,14 +123,9 @@ __device__ __forceinline__ uint32_t perm_at(const DataConst& d, uint32_t pivot, 
 // ---------------------------------------------------------------------------
 // antiphase
 
-// utils::antiphase  src/utilities.tpp:567-577 of the Hermitian product
-// z = re + i im: polar(1, -atan2(im, re)).  Literal keeps the reference's
-// atan2/cos/sin; otherwise the identical closed form conj(z)/|z| (cos(atan2(y,
-// x)) = x/|z|, sin(atan2(y,x)) = y/|z|), scaled so |z|^2 neither underflows nor
-// overflows, with the two cases atan2 fixes exactly kept exact: z real and
-// positive gives 1, z = 0 gives +-1 by the sign of re.
-// one out-of-line copy: the slot loops that call it are unrolled, and eight
-// inlined atan2/sincos bodies would crowd the instruction cache
+// one out-of-line copy of the reference's formula: the slot loops that call
+// it are unrolled, and eight inlined atan2/sincos bodies would crowd the
+// instruction cache
 __device__ __noinline__ double2 antiphase_literal(double re, double im) {
   double th = -1.0 * atan2(im, re);
   double s, c;
@@ -138,6 +133,12 @@ __device__ __noinline__ double2 antiphase_literal(double re, double im) {
   return make_double2(1.0 * c, 1.0 * s);
 }
 
+// utils::antiphase  src/utilities.tpp:567-577 of the Hermitian product
+// z = re + i im: polar(1, -atan2(im, re)).  Literal keeps the reference's
+// atan2/cos/sin; otherwise the identical closed form conj(z)/|z| (cos(atan2(y,
+// x)) = x/|z|, sin(atan2(y,x)) = y/|z|), rescaled when |z|^2 would underflow or
+// overflow, with the two cases atan2 fixes exactly kept exact: z real and
+// positive gives 1, z = 0 gives +-1 by the sign of re.
 template <typename Real, bool Literal>
 __device__ __forceinline__ typename Cplx<Real>::T antiphase_of(Real re, Real im) {
   typename Cplx<Real>::T e;
@@ -146,11 +147,20 @@ __device__ __forceinline__ typename Cplx<Real>::T antiphase_of(Real re, Real im)
     e.x = (Real)t.x;
     e.y = (Real)t.y;
   } else {
-    const Real m = fmax(fabs(re), fabs(im));
+    constexpr Real lo = sizeof(Real) == 4 ? (Real)1e-36 : (Real)1e-300;
+    constexpr Real hi = sizeof(Real) == 4 ? (Real)1e36 : (Real)1e300;
+    const Real r2 = re * re + im * im;
     if (im == (Real)0 && re > (Real)0) {
       e.x = 1;
       e.y = 0;
-    } else if (m > (Real)0) {
+    } else if (r2 > lo && r2 < hi) {
+      Real inv;
+      if constexpr (sizeof(Real) == 4) inv = __builtin_amdgcn_rsqf(r2);  // v_rsq_f32, 1 ulp
+      else inv = 1.0 / sqrt(r2);
+      e.x = re * inv;
+      e.y = -im * inv;
+    } else if (re != (Real)0 || im != (Real)0) {
+      const Real m = fmax(fabs(re), fabs(im));
       const Real a = re / m, b = im / m;
       const Real inv = (Real)1 / sqrt(a * a + b * b);
       e.x = a * inv;
@@ -537,13 +547,15 @@ struct LaunchArgs {
   ErrRec* err;
   int no_move;
   int use_proposal;
+  ProposalConst prop;
 };
 
 // ---------------------------------------------------------------------------
 // phase 1 for one point
 
+// stage (a) and the node subscript of stage (b): q, tau, Ridx/invRidx, node
 template <bool Fast>
-__device__ inline void locate(const GridConst& g, const double* Q, bool no_move, Located& L) {
+__device__ inline void fold_node(const GridConst& g, const double* Q, bool no_move, Located& L) {
   const Tol tbz{g.tol_bz_rel, g.tol_bz_abs};
   const Tol tdef{g.tol_def_rel, g.tol_def_abs};
   L.tau[0] = L.tau[1] = L.tau[2] = 0;
@@ -554,11 +566,6 @@ __device__ inline void locate(const GridConst& g, const double* Q, bool no_move,
   L.nkept = 0;
   L.pivot = 0;
   L.status = BRILLE_PT_OK;
-#pragma unroll
-  for (int s = 0; s < 8; ++s) {
-    L.idx[s] = 0;
-    L.w[s] = 0.0;
-  }
   if (no_move) {
     L.q[0] = Q[0]; L.q[1] = Q[1]; L.q[2] = Q[2];
   } else {
@@ -572,11 +579,46 @@ __device__ inline void locate(const GridConst& g, const double* Q, bool no_move,
   uint64_t node = 0;
   if (!node_subscript<Fast>(g, x, &node, tdef)) { L.status = BRILLE_PT_NULL_NODE; return; }
   L.node = (uint32_t)node;
+}
+
+// the rest of stage (b) for a folded point with a usable node: slots, pivot
+// hint: a proposed tetrahedron of the node (kNoTet: none).  It is taken only
+// when the reference's scan would stop there too: the circumsphere shortcut
+// lets it through and every barycentric weight exceeds `margin` (a point that
+// deep inside one tetrahedron is outside every other by more than the
+// comparison tolerance, so no earlier tetrahedron passes the containment test).
+// Otherwise the exact scan runs.  *verified tells which happened.
+__device__ inline void node_slots(const GridConst& g, Located& L, uint32_t hint = kNoTet, double margin = 0.0,
+                                  bool* verified = nullptr) {
+  const Tol tdef{g.tol_def_rel, g.tol_def_abs};
+  double x[3];
+  mat_vec(x, g.outer_to_xyz, L.q);  // brille::get_xyz
+  const uint32_t node = L.node;
+  L.tet = kNoTet;
+  L.nslot = 0;
+  L.nkept = 0;
+  L.pivot = 0;
+#pragma unroll
+  for (int s = 0; s < 8; ++s) {
+    L.idx[s] = 0;
+    L.w[s] = 0.0;
+  }
   if (g.node_type[node] == BRILLE_NODE_CUBE) {
     cube_slots(g, g.node_index[node], x, L.idx, L.w, tdef);
     L.nslot = 8;
   } else {
-    L.tet = poly_slots(g, g.node_index[node], x, L.idx, L.w, tdef);
+    bool ok = false;
+    if (hint != kNoTet && tet_might_contain(g, hint, x, tdef) >= 0.0) {
+      double ww[4];
+      tet_contains(g, hint, x, ww, false, tdef);
+      ok = ww[0] > margin && ww[1] > margin && ww[2] > margin && ww[3] > margin;
+      if (ok) {
+        tet_slots(g, hint, ww, L.idx, L.w, tdef);
+        L.tet = hint;
+      }
+    }
+    if (verified) *verified = ok;
+    if (!ok) L.tet = poly_slots(g, g.node_index[node], x, L.idx, L.w, tdef);
     L.nslot = L.tet == kNoTet ? 0 : 4;
   }
   // pivot = first kept slot; unused tetrahedron slots repeat it with weight 0
@@ -589,6 +631,12 @@ __device__ inline void locate(const GridConst& g, const double* Q, bool no_move,
     if (s >= L.nslot) L.idx[s] = L.pivot;
   }
   if (L.nkept < 1) { L.status = BRILLE_PT_NOT_FOUND; L.nslot = 0; }
+}
+
+template <bool Fast>
+__device__ inline void locate(const GridConst& g, const double* Q, bool no_move, Located& L) {
+  fold_node<Fast>(g, Q, no_move, L);
+  if (L.status == BRILLE_PT_OK) node_slots(g, L);
 }
 
 // aux outputs of one point (weights in FP64 whatever the sums' precision)
@@ -618,70 +666,59 @@ __device__ inline void write_aux(const brille_aux_out& aux, uint64_t p, const Lo
 }
 
 // ---------------------------------------------------------------------------
-// the kernel.  L selects phase 2; SC/SV are the compile-time row sizes of the
-// specialised layouts (ignored by LAYOUT_GENERIC).  Plain (Fast = false) is
-// the literal restatement: exact fold-in and location, atan2/sincos
-// antiphases.
+// shared pieces of the tile kernels.  L selects phase 2; SC/SV are the
+// compile-time row sizes of the specialised layouts (ignored by
+// LAYOUT_GENERIC).
 
-template <typename Real, bool Fast, int L, int SC, int SV>
-__global__ __launch_bounds__(TILE) void k_ir_interpolate(GridConst g, DataConst d, LaunchArgs a) {
-  using C = typename Cplx<Real>::T;
-  constexpr int NPH = L == LAYOUT_GAMMA ? SC / 3 : (L == LAYOUT_GENERIC ? 4 : 1);
-  constexpr int NQ = L == LAYOUT_GENERIC ? 3 * TILE : 1;
-  __shared__ PointRec<Real> s_rec[TILE];
-  __shared__ C s_ph[TILE * NPH];
-  __shared__ double s_q[NQ];
-  const uint64_t tile0 = (uint64_t)blockIdx.x * TILE;
-  const uint32_t tid = threadIdx.x;
-  const uint64_t p = tile0 + tid;
-  const bool want_phase = d.vec_rot == BRILLE_ROT_GAMMA && d.ec[1] > 0;
-  const uint32_t n_atoms = d.ec[1] / 3u;
-  const bool ph_in_lds = L == LAYOUT_GAMMA || (want_phase && n_atoms <= 4);
-  {
-    PointRec<Real>& r = s_rec[tid];
-    r.nslot = 0;
-    if (p < a.n) {
-      double Qp[3] = {a.Q[3 * p], a.Q[3 * p + 1], a.Q[3 * p + 2]};
-      Located P;
-      locate<Fast>(g, Qp, a.no_move, P);
-      write_aux(a.aux, p, P);
-      if (P.status != BRILLE_PT_OK) {
-        unsigned long long code = ((unsigned long long)(a.base + p) << 3) | (unsigned long long)P.status;
-        atomicMin(&a.err->first_bad, code);
-        atomicAdd(&a.err->count, 1ull);
-      } else {
+template <int L, int SC, int TB = TILE>
+struct TileShape {
+  static constexpr int NPH = L == LAYOUT_GAMMA ? SC / 3 : (L == LAYOUT_GENERIC ? 4 : 1);  // phases per point
+  static constexpr int NQ = L == LAYOUT_GENERIC ? 3 * TB : 1;                              // q kept in LDS
+};
+
+// a located point's phase-2 record (plus phases, and q for the general layout)
+template <typename Real, int L, int SC>
+__device__ __forceinline__ void store_record(const DataConst& d, const Located& P, PointRec<Real>& r,
+                                             typename Cplx<Real>::T* s_ph, double* s_q, uint32_t slot,
+                                             bool ph_in_lds, uint32_t n_atoms) {
+  constexpr int NPH = TileShape<L, SC>::NPH;
 #pragma unroll
-        for (int s = 0; s < 8; ++s) {
-          r.idx[s] = P.idx[s];
-          r.w[s] = (Real)P.w[s];
-        }
-        r.pivot = P.pivot;
-        r.rot = (uint16_t)P.rot;
-        r.invrot = (uint16_t)P.invrot;
-        r.nslot = (uint8_t)P.nslot;
-        if constexpr (L == LAYOUT_GAMMA) {
-#pragma unroll
-          for (int k = 0; k < NPH; ++k) s_ph[NPH * tid + k] = gamma_phase<Real>(d, P.q, P.invrot, k);
-        } else if constexpr (L == LAYOUT_GENERIC) {
-          s_q[3 * tid] = P.q[0]; s_q[3 * tid + 1] = P.q[1]; s_q[3 * tid + 2] = P.q[2];
-          if (ph_in_lds)
-            for (uint32_t k = 0; k < n_atoms; ++k) s_ph[NPH * tid + k] = gamma_phase<Real>(d, P.q, P.invrot, k);
-        }
-      }
-    }
+  for (int s = 0; s < 8; ++s) {
+    r.idx[s] = P.idx[s];
+    r.w[s] = (Real)P.w[s];
   }
-  __syncthreads();
-  const uint64_t left = a.n - tile0;
-  const uint32_t npts = left < (uint64_t)TILE ? (uint32_t)left : (uint32_t)TILE;
+  r.pivot = P.pivot;
+  r.rot = (uint16_t)P.rot;
+  r.invrot = (uint16_t)P.invrot;
+  r.nslot = (uint8_t)P.nslot;
+  if constexpr (L == LAYOUT_GAMMA) {
+#pragma unroll
+    for (int k = 0; k < NPH; ++k) s_ph[NPH * slot + k] = gamma_phase<Real>(d, P.q, P.invrot, k);
+  } else if constexpr (L == LAYOUT_GENERIC) {
+    s_q[3 * slot] = P.q[0]; s_q[3 * slot + 1] = P.q[1]; s_q[3 * slot + 2] = P.q[2];
+    if (ph_in_lds)
+      for (uint32_t k = 0; k < n_atoms; ++k) s_ph[NPH * slot + k] = gamma_phase<Real>(d, P.q, P.invrot, k);
+  }
+}
+
+// phase 2 over the npts records of a tile of TB points (TB threads): point pl
+// writes output row out_row[pl] (or row0 + pl when out_row is null)
+template <typename Real, bool Fast, int L, int SC, int SV, int TB = TILE>
+__device__ __forceinline__ void tile_sums(const GridConst& g, const DataConst& d, const PointRec<Real>* s_rec,
+                                          const typename Cplx<Real>::T* s_ph, const double* s_q,
+                                          const uint32_t* out_row, uint64_t row0, uint32_t npts, bool ph_in_lds,
+                                          void* vals_, void* vecs_) {
+  using C = typename Cplx<Real>::T;
+  constexpr int NPH = TileShape<L, SC>::NPH;
   const uint32_t B = d.B;
   const uint32_t units = npts * B;
-  Real* vals = (Real*)a.vals;
-  C* vecs = (C*)a.vecs;
+  Real* vals = (Real*)vals_;
+  C* vecs = (C*)vecs_;
   const bool cart = d.vec_lu == BRILLE_LU_ANGSTROM;
-  for (uint32_t u = tid; u < units; u += TILE) {
+  for (uint32_t u = threadIdx.x; u < units; u += TB) {
     const uint32_t pl = u / B, b = u - pl * B;
     const PointRec<Real>& r = s_rec[pl];
-    const uint64_t row = (tile0 + pl) * B + b;
+    const uint64_t row = ((out_row ? (uint64_t)out_row[pl] : row0 + pl)) * B + b;
     Real* ov = vals + row * d.sv;
     C* oc = vecs + row * d.sc;
     if (r.nslot == 0) {
@@ -698,6 +735,52 @@ __global__ __launch_bounds__(TILE) void k_ir_interpolate(GridConst g, DataConst 
       unit_generic<Real, !Fast>(g, d, r, s_q + 3 * pl, b, ov, oc, ph_in_lds ? s_ph + NPH * pl : nullptr);
     }
   }
+}
+
+__device__ __forceinline__ void report_point(ErrRec* err, uint64_t index, int status) {
+  atomicMin(&err->first_bad, (index << 3) | (unsigned long long)status);
+  atomicAdd(&err->count, 1ull);
+}
+
+// ---------------------------------------------------------------------------
+// the fused kernel: one tile of TILE input points, both phases.  Plain
+// (Fast = false) is the literal restatement: exact fold-in and location,
+// atan2/sincos antiphases.
+
+template <typename Real, bool Fast, int L, int SC, int SV>
+__global__ __launch_bounds__(TILE) void k_ir_interpolate(GridConst g, DataConst d, LaunchArgs a) {
+  using C = typename Cplx<Real>::T;
+  using S = TileShape<L, SC>;
+  __shared__ PointRec<Real> s_rec[TILE];
+  __shared__ C s_ph[TILE * S::NPH];
+  __shared__ double s_q[S::NQ];
+  const uint64_t tile0 = (uint64_t)blockIdx.x * TILE;
+  const uint32_t tid = threadIdx.x;
+  const uint64_t p = tile0 + tid;
+  const bool want_phase = d.vec_rot == BRILLE_ROT_GAMMA && d.ec[1] > 0;
+  const uint32_t n_atoms = d.ec[1] / 3u;
+  const bool ph_in_lds = L == LAYOUT_GAMMA || (want_phase && n_atoms <= 4);
+  {
+    PointRec<Real>& r = s_rec[tid];
+    r.nslot = 0;
+    const bool live = p < a.n;
+    Located P;
+    P.status = BRILLE_PT_OK;
+    if (live) {
+      double Qp[3] = {a.Q[3 * p], a.Q[3 * p + 1], a.Q[3 * p + 2]};
+      fold_node<Fast>(g, Qp, a.no_move, P);
+    }
+    if (live && P.status == BRILLE_PT_OK) node_slots(g, P);
+    if (live) {
+      write_aux(a.aux, p, P);
+      if (P.status != BRILLE_PT_OK) report_point(a.err, a.base + p, P.status);
+      else store_record<Real, L, SC>(d, P, r, s_ph, s_q, tid, ph_in_lds, n_atoms);
+    }
+  }
+  __syncthreads();
+  const uint64_t left = a.n - tile0;
+  const uint32_t npts = left < (uint64_t)TILE ? (uint32_t)left : (uint32_t)TILE;
+  tile_sums<Real, Fast, L, SC, SV>(g, d, s_rec, s_ph, s_q, nullptr, tile0, npts, ph_in_lds, a.vals, a.vecs);
 }
 
 }  // namespace brille_amd

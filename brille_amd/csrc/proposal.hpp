@@ -1,0 +1,171 @@
+// Learned tetrahedron proposal: inference on the matrix cores.
+//
+// The network (3 -> 128 -> 128 -> K, ReLU) proposes which tetrahedron of a
+// polyhedron node holds a point.  The proposal is only a hint: the tile
+// kernels verify it with the reference's own FP64 predicates and run the
+// exact scan of PolyNode::indices_weights (src/trellis_node.hpp:275-308) when
+// it does not pass, so the chosen tetrahedron, the indices and the weights are
+// always the reference's.
+//
+// One wave evaluates 16 points at a time:
+//   layer 1 (3 -> 128, FP32) is computed straight into the A operands of
+//     v_mfma_f32_16x16x32_bf16: lane l holds H1[i = l%16][k = 32 kb + 8 (l/16) + e];
+//   layer 2 (128 -> 128) is 8 column blocks x 4 k-blocks of that MFMA, with
+//     the bf16 rows of W2 as B operands (B[k][j] = W2[j][k], lane l holds
+//     k = 32 kb + 8 (l/16) + e, j = 16 nb + l%16); D[i][j] lands in lane l as
+//     i = 4 (l/16) + r, j = 16 nb + l%16, r = 0..3;
+//   layer 3 is evaluated only for the candidate tetrahedra of each point: the
+//     16 lanes of a group hold one point's 128 hidden units (8 per lane) and
+//     reduce W3[c] . h2 with four xor-shuffles.  W3 rows are stored
+//     lane-major so that each lane reads its 8 weights with one 16-byte load.
+#pragma once
+#include "exact_path.hpp"
+
+namespace brille_amd {
+
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float floatx4_t __attribute__((ext_vector_type(4)));
+
+// network input of a point: its Cartesian coordinates scaled into [-1, 1]
+__device__ __forceinline__ void prop_features(const ProposalConst& pc, const double* x, float* f) {
+#pragma unroll
+  for (int d = 0; d < 3; ++d) f[d] = (float)((x[d] - (double)pc.lo[d]) * (double)pc.scale[d] - 1.0);
+}
+
+// Proposals for the n <= 16 points of a chunk (feat: 3 floats per point,
+// t0/nc: first candidate and candidate count).  Called by a whole wave with
+// wave-uniform arguments; out[i] receives the first candidate of largest logit.
+// Out of line: inlined into a loop over chunks, its per-lane layer-1 weights
+// would be hoisted and held across iterations (over a hundred registers).
+__device__ __noinline__ void propose16(const ProposalConst& pc, const float* feat, const uint32_t* t0, const uint32_t* nc,
+                                 uint32_t n, uint32_t* out) {
+  const uint32_t l = threadIdx.x & 63u, m = l & 15u, grp = l >> 4;
+  float x0 = 0.f, x1 = 0.f, x2 = 0.f;
+  if (m < n) {
+    x0 = feat[3 * m];
+    x1 = feat[3 * m + 1];
+    x2 = feat[3 * m + 2];
+  }
+  bf16x8_t a[4];
+#pragma unroll
+  for (int kb = 0; kb < 4; ++kb)
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int k = kb * 32 + 8 * (int)grp + e;
+      float h = pc.W1[3 * k] * x0;
+      h += pc.W1[3 * k + 1] * x1;
+      h += pc.W1[3 * k + 2] * x2;
+      h += pc.b1[k];
+      a[kb][e] = (__bf16)(h > 0.f ? h : 0.f);
+    }
+  floatx4_t acc[8];
+#pragma unroll
+  for (int nb = 0; nb < 8; ++nb) {
+    acc[nb] = floatx4_t{0.f, 0.f, 0.f, 0.f};
+    const bf16x8_t* w = reinterpret_cast<const bf16x8_t*>(pc.W2 + (size_t)(nb * 16 + m) * kPropH + 8 * grp);
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb) acc[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[kb], w[kb * 4], acc[nb], 0, 0, 0);
+    const float bj = pc.b2[nb * 16 + m];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const float v = acc[nb][r] + bj;
+      acc[nb][r] = v > 0.f ? v : 0.f;
+    }
+  }
+  uint32_t ncr[4], t0r[4], bestc[4];
+  float best[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const uint32_t pi = 4 * grp + r;
+    ncr[r] = pi < n ? nc[pi] : 0u;
+    t0r[r] = pi < n ? t0[pi] : 0u;
+    bestc[r] = t0r[r];
+    best[r] = -3.0e38f;
+  }
+  uint32_t maxc = 0;
+  for (uint32_t pi = 0; pi < n; ++pi) maxc = nc[pi] > maxc ? nc[pi] : maxc;
+  for (uint32_t s = 0; s < maxc; ++s) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const bool act = s < ncr[r];
+      const uint32_t c = act ? t0r[r] + s : 0u;
+      float part = 0.f;
+      if (act) {
+        const bf16x8_t w = *reinterpret_cast<const bf16x8_t*>(pc.W3 + (size_t)c * kPropH + 8 * m);
+#pragma unroll
+        for (int nb = 0; nb < 8; ++nb) part += (float)w[nb] * acc[nb][r];
+      }
+      part += __shfl_xor(part, 1);
+      part += __shfl_xor(part, 2);
+      part += __shfl_xor(part, 4);
+      part += __shfl_xor(part, 8);
+      if (act) {
+        const float z = part + pc.b3[c];
+        if (z > best[r]) {
+          best[r] = z;
+          bestc[r] = c;
+        }
+      }
+    }
+  }
+  if (m == 0) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const uint32_t pi = 4 * grp + r;
+      if (pi < n) out[pi] = bestc[r];
+    }
+  }
+}
+
+// LDS queue of the points of one tile that need the surface search
+template <int TB>
+struct PropQueue {
+  uint32_t count, tried, verified, pad;
+  float feat[3 * TB];
+  uint32_t t0[TB], nc[TB], out[TB];
+};
+
+// Block-wide: compacts the lanes with `want` into the queue (wave ballot and
+// prefix count, one LDS atomic per wave), runs the network over the queue 16
+// points per wave, and returns the proposal of the calling lane (kNoTet when
+// it did not want one).  Every thread of the block must call it.
+template <int TB>
+__device__ inline uint32_t propose_block(const GridConst& g, const ProposalConst& pc, PropQueue<TB>& pq, bool want,
+                                         const double* q, uint32_t node) {
+  if (threadIdx.x == 0) pq.count = 0;
+  __syncthreads();
+  const uint32_t lane = threadIdx.x & 63u;
+  const unsigned long long mask = __ballot(want);
+  const uint32_t rank = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+  uint32_t base = 0;
+  if (mask) {
+    const uint32_t leader = (uint32_t)__ffsll((long long)mask) - 1u;
+    if (lane == leader) base = atomicAdd(&pq.count, (uint32_t)__popcll(mask));
+    base = __shfl(base, (int)leader);
+  }
+  const uint32_t slot = base + rank;
+  if (want) {
+    double x[3];
+    mat_vec(x, g.outer_to_xyz, q);
+    float f[3];
+    prop_features(pc, x, f);
+    pq.feat[3 * slot] = f[0];
+    pq.feat[3 * slot + 1] = f[1];
+    pq.feat[3 * slot + 2] = f[2];
+    const uint32_t p = g.node_index[node];
+    const uint32_t a = g.poly_off[p];
+    pq.t0[slot] = a;
+    pq.nc[slot] = g.poly_off[p + 1] - a;
+  }
+  __syncthreads();
+  const uint32_t n = pq.count;
+  const uint32_t wave = threadIdx.x >> 6;
+  for (uint32_t c0 = wave * 16u; c0 < n; c0 += (TB / 64) * 16u) {
+    const uint32_t k = n - c0 < 16u ? n - c0 : 16u;
+    propose16(pc, pq.feat + 3 * c0, pq.t0 + c0, pq.nc + c0, k, pq.out + c0);
+  }
+  __syncthreads();
+  return want ? pq.out[slot] : kNoTet;
+}
+
+}  // namespace brille_amd

@@ -77,7 +77,7 @@ int check_rotation_modes(const brille_grid* G) {
 using TileKernel = void (*)(GridConst, DataConst, LaunchArgs);
 
 // which specialised phase 2 the payload admits
-static int payload_layout(const DataConst& d) {
+int payload_layout(const DataConst& d) {
   const bool scalar_values = d.ev[1] == 0 && d.ev[2] == 0;
   if (d.n_keys || !scalar_values) return LAYOUT_GENERIC;
   if (d.ec[1] == 0 && d.ec[2] == 0) return LAYOUT_SCALAR;
@@ -124,6 +124,14 @@ static void launch_sel(const brille_grid* G, const LaunchArgs& a, int precision,
   hipLaunchKernelGGL(k, dim3((unsigned)blocks), dim3(TILE), 0, s, G->g, G->d, a);
 }
 
+// every kernel of one call on device buffers
+static int dispatch(brille_grid* G, const LaunchArgs& a, const brille_run_opts& o, hipStream_t s) {
+  const bool fast = o.kernel != BRILLE_KERNEL_PLAIN;
+  if (fast && binned_applies(G, a, o.kernel)) return launch_binned(G, a, o.precision, s, nullptr);
+  launch_sel(G, a, o.precision, fast, s);
+  return BRILLE_OK;
+}
+
 int report_point_error(const ErrRec& e, const double* Q, uint64_t n, bool host_mem) {
   const uint64_t idx = e.first_bad >> 3;
   const int st = (int)(e.first_bad & 7ull);
@@ -154,6 +162,16 @@ int report_point_error(const ErrRec& e, const double* Q, uint64_t n, bool host_m
   }
 }
 
+// proposals tried / verified by the call that just ran on stream s
+static int read_proposal_stats(brille_grid* G, hipStream_t s) {
+  unsigned long long st[2] = {0, 0};
+  HIP_TRY(hipMemcpyAsync(st, G->prop_stats, sizeof(st), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  G->last_tried = st[0];
+  G->last_hits = st[1];
+  return BRILLE_OK;
+}
+
 int run_interpolate(brille_grid* G, const double* Q, uint64_t n, const brille_run_opts* opts_in, void* values_out,
                     void* vectors_out, const brille_aux_out* aux_in, hipStream_t s, brille_kernel_times* times,
                     uint32_t iters) {
@@ -171,6 +189,10 @@ int run_interpolate(brille_grid* G, const double* Q, uint64_t n, const brille_ru
   if (!Q || !values_out || !vectors_out) return fail(BRILLE_ERR_INVALID_ARGUMENT, "null input or output pointer");
   if (opts.use_proposal && !G->prop_trained)
     return fail(BRILLE_ERR_INVALID_ARGUMENT, "use_proposal requires a trained proposal (brille_proposal_train)");
+  if (opts.use_proposal && (opts.kernel == BRILLE_KERNEL_PLAIN || opts.kernel == BRILLE_KERNEL_TILE))
+    return fail(BRILLE_ERR_INVALID_ARGUMENT, "the learned proposal runs in the binned pipeline (kernel AUTO, TUNED or BINNED)");
+  if (opts.use_proposal && aux_in && aux_in->tau)
+    return fail(BRILLE_ERR_INVALID_ARGUMENT, "the binned pipeline does not report tau");
   const bool fast = opts.kernel != BRILLE_KERNEL_PLAIN;
   const size_t rs = opts.precision == BRILLE_FP32 ? 4 : 8;
   const size_t vbytes = (size_t)G->d.B * G->d.sv * rs;
@@ -180,6 +202,8 @@ int run_interpolate(brille_grid* G, const double* Q, uint64_t n, const brille_ru
   LaunchArgs a{};
   a.no_move = opts.no_move;
   a.use_proposal = opts.use_proposal;
+  a.prop = G->prop;
+  if (opts.use_proposal) HIP_TRY(hipMemsetAsync(G->prop_stats, 0, 2 * sizeof(unsigned long long), s));
   a.err = G->err;
   if (opts.memory == BRILLE_MEM_DEVICE) {
     a.Q = Q;
@@ -190,27 +214,48 @@ int run_interpolate(brille_grid* G, const double* Q, uint64_t n, const brille_ru
     a.aux = aux;
     hipLaunchKernelGGL(k_err_init2, dim3(1), dim3(1), 0, s, G->err);
     if (times) {
-      hipEvent_t e0, e1;
-      HIP_TRY(hipEventCreate(&e0));
-      HIP_TRY(hipEventCreate(&e1));
-      HIP_TRY(hipEventRecord(e0, s));
-      for (uint32_t it = 0; it < std::max(iters, 1u); ++it) launch_sel(G, a, opts.precision, fast, s);
-      HIP_TRY(hipEventRecord(e1, s));
-      HIP_TRY(hipGetLastError());
-      HIP_TRY(hipEventSynchronize(e1));
-      float ms = 0;
-      HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-      (void)hipEventDestroy(e0);
-      (void)hipEventDestroy(e1);
       std::memset(times, 0, sizeof(*times));
-      times->n_kernels = 1;
-      std::snprintf(times->names[0], sizeof(times->names[0]), "k_ir_interpolate");
-      times->ms[0] = ms / (float)std::max(iters, 1u);
-      times->total_ms = times->ms[0];
+      const uint32_t it_n = std::max(iters, 1u);
+      if (fast && binned_applies(G, a, opts.kernel)) {
+        float st[3] = {0.f, 0.f, 0.f};
+        for (uint32_t it = 0; it < it_n; ++it) {
+          rc = launch_binned(G, a, opts.precision, s, st);
+          if (rc != BRILLE_OK) return rc;
+        }
+        const char* nm[3] = {"k_bin_fold", "k_bin_bucket", "k_bin_sums"};
+        times->n_kernels = 3;
+        for (int k = 0; k < 3; ++k) {
+          std::snprintf(times->names[k], sizeof(times->names[k]), "%s", nm[k]);
+          times->ms[k] = st[k] / (float)it_n;
+          times->total_ms += times->ms[k];
+        }
+      } else {
+        hipEvent_t e0, e1;
+        HIP_TRY(hipEventCreate(&e0));
+        HIP_TRY(hipEventCreate(&e1));
+        HIP_TRY(hipEventRecord(e0, s));
+        for (uint32_t it = 0; it < it_n; ++it) launch_sel(G, a, opts.precision, fast, s);
+        HIP_TRY(hipEventRecord(e1, s));
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipEventSynchronize(e1));
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        times->n_kernels = 1;
+        std::snprintf(times->names[0], sizeof(times->names[0]), "k_ir_interpolate");
+        times->ms[0] = ms / (float)it_n;
+        times->total_ms = times->ms[0];
+      }
     } else {
-      launch_sel(G, a, opts.precision, fast, s);
+      rc = dispatch(G, a, opts, s);
+      if (rc != BRILLE_OK) return rc;
     }
     HIP_TRY(hipGetLastError());
+    if (opts.use_proposal) {
+      rc = read_proposal_stats(G, s);
+      if (rc != BRILLE_OK) return rc;
+    }
     if (!opts.check && !times) return BRILLE_OK;
     ErrRec e;
     HIP_TRY(hipMemcpyAsync(&e, G->err, sizeof(e), hipMemcpyDeviceToHost, s));
@@ -259,12 +304,17 @@ int run_interpolate(brille_grid* G, const double* Q, uint64_t n, const brille_ru
     for (int i = 0; i < 10; ++i)
       if (*auxp[i]) *dauxp[i] = st + slots[k++].off;
     a.aux = da;
-    launch_sel(G, a, opts.precision, fast, s);
+    rc = dispatch(G, a, opts, s);
+    if (rc != BRILLE_OK) return rc;
     HIP_TRY(hipGetLastError());
     for (size_t i = 1; i < slots.size(); ++i)
       HIP_TRY(hipMemcpyAsync(static_cast<char*>(slots[i].host) + c0 * slots[i].per, st + slots[i].off, m * slots[i].per,
                              hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
+  }
+  if (opts.use_proposal) {
+    rc = read_proposal_stats(G, s);
+    if (rc != BRILLE_OK) return rc;
   }
   ErrRec e;
   HIP_TRY(hipMemcpyAsync(&e, G->err, sizeof(e), hipMemcpyDeviceToHost, s));
@@ -303,11 +353,6 @@ int build_fast_tables(const brille_grid_desc* d, brille_grid* G) {
   return BRILLE_OK;
 }
 
-void free_proposal(brille_grid* G) {
-  for (void* p : G->prop_allocs) (void)hipFree(p);
-  G->prop_allocs.clear();
-  G->prop_trained = false;
-}
 
 }  // namespace brille_amd
 
@@ -321,11 +366,5 @@ int brille_time_ir_interpolate_at(brille_grid* G, const double* Q, uint64_t n, c
   return run_interpolate(G, Q, n, opts, values_out, vectors_out, nullptr, static_cast<hipStream_t>(stream), times, iters);
 }
 
-int brille_proposal_stats(const brille_grid* G, uint64_t* tried, uint64_t* hits) {
-  if (!G) return fail(BRILLE_ERR_INVALID_ARGUMENT, "null grid");
-  if (tried) *tried = G->last_tried;
-  if (hits) *hits = G->last_hits;
-  return BRILLE_OK;
-}
 
 }  // extern "C"

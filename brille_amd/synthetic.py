@@ -118,3 +118,113 @@ def q_points(cfg, n=None, stream=2):
     c = CONFIGS[cfg]
     n = c["n_q"] if n is None else n
     return rng(cfg, stream).uniform(-c["box"], c["box"], size=(int(n), 3))
+
+
+# ---------------------------------------------------------------------------
+# configuration 2: many uneven grids for the batched launch
+
+BATCH_SPACEGROUPS = ("221", "139", "62", "14", "194")
+BATCH_MONOCLINIC_BETA = (95.0, 115.0)  # beta is drawn from this range
+BATCH_PAYLOAD_CAP = 256 << 20          # bytes of complex128 eigenvectors per grid (caps V for many atoms)
+
+
+def _batch_lattice_parameters(sg, r):
+    a, b, c = r.uniform(3.0, 8.0, 3)
+    if sg == "221":
+        return (a, a, a), (90.0, 90.0, 90.0)
+    if sg == "139":
+        return (a, a, c), (90.0, 90.0, 90.0)
+    if sg == "62":
+        return (a, b, c), (90.0, 90.0, 90.0)
+    if sg == "14":
+        return (a, b, c), (90.0, float(r.uniform(*BATCH_MONOCLINIC_BETA)), 90.0)
+    return (a, a, c), (90.0, 90.0, 120.0)
+
+
+def _orbit(sym, x):
+    pos = []
+    for k in range(sym.W.shape[0]):
+        p = (sym.W[k] @ x + sym.w[k]) % 1.0
+        p[np.isclose(p, 1.0, atol=1e-9)] = 0.0
+        if not any(np.allclose(p, q, atol=1e-8) for q in pos):
+            pos.append(p)
+    return pos
+
+
+def _batch_basis(sg, target, r):
+    """A symmetry-closed basis of at most 24 atoms and about `target` atoms:
+    whole orbits of special and general positions."""
+    sym = _b.Symmetry(sg)
+    cands = [lambda: np.zeros(3), lambda: np.full(3, 0.5), lambda: np.array([0.0, 0.5, 0.5]),
+             lambda: np.array([0.5, 0.0, 0.0]), lambda: np.full(3, 0.25),
+             lambda: np.array([r.uniform(0.05, 0.45), 0.0, 0.0]),
+             lambda: np.full(3, r.uniform(0.05, 0.45)),
+             lambda: np.array([r.uniform(0.05, 0.45), r.uniform(0.05, 0.45), 0.0]),
+             lambda: np.array([r.uniform(0.05, 0.45), 0.25, r.uniform(0.05, 0.45)]),
+             lambda: r.uniform(0.05, 0.95, 3)]
+    pos, typ = [], []
+    for attempt in range(400):
+        if len(pos) >= target:
+            break
+        orb = _orbit(sym, cands[r.integers(len(cands))]())
+        if len(pos) + len(orb) > 24 or any(np.allclose(p, q, atol=1e-6) for p in orb for q in pos):
+            continue
+        pos.extend(orb)
+        typ.extend([len(set(typ))] * len(orb))
+    if not pos:
+        pos, typ = [np.zeros(3)], [0]
+    return np.array(pos), np.array(typ)
+
+
+def batch_grids(n_grids=64, v_range=(2e3, 2e5), n_q=10_000_000, box=3.0, cfg=2, verbose=None):
+    """Configuration 2: n_grids filled BZTrellisQdc of uneven vertex and mode
+    counts and their Q.  Returns (grids, Q, offsets): grid g owns
+    Q[offsets[g]:offsets[g+1]].
+
+    Per grid: space group cycling BATCH_SPACEGROUPS, lattice lengths U(3,8) A
+    (monoclinic beta U(95,115) deg), a symmetry-closed basis of 1-24 atoms
+    (target log-uniform), 3A modes, a node volume solved (one trial build)
+    for a vertex count log-uniform in v_range, capped so the complex128
+    eigenvectors stay under BATCH_PAYLOAD_CAP bytes.  Payload: smooth seeded
+    eigenvalues and random unit eigenvectors (not symmetrised: the batch
+    exercises scheduling over uneven grids, not the physics).  Q: n_q points
+    uniform in [-box, box]^3 split across grids with Dirichlet(0.3) weights."""
+    r = rng(cfg, 0)
+    grids = []
+    for gi in range(n_grids):
+        sg = BATCH_SPACEGROUPS[gi % len(BATCH_SPACEGROUPS)]
+        lengths, angles = _batch_lattice_parameters(sg, r)
+        target_atoms = int(round(np.exp(r.uniform(0.0, np.log(24.0)))))
+        pos, typ = _batch_basis(sg, target_atoms, r)
+        A = len(pos)
+        B = 3 * A
+        v_target = float(np.exp(r.uniform(np.log(v_range[0]), np.log(v_range[1]))))
+        v_target = min(v_target, BATCH_PAYLOAD_CAP / (B * 3 * A * 16.0))
+        v_target = max(v_target, min(v_range[0], 200.0))
+        lat = _b.Direct(np.array(lengths), np.array(angles), sg, _b.Basis(pos, typ))
+        bz = _b.BrillouinZone(lat)
+        vir = bz.ir_polyhedron.volume
+        trial_nv = vir / 300.0
+        trial = _b.BZTrellisQdc(bz, trial_nv)
+        nv = trial_nv * (trial.rlu.shape[0] / v_target) ** 1.0
+        g = _b.BZTrellisQdc(bz, nv)
+        rlu = g.rlu
+        V = rlu.shape[0]
+        rv = rng(cfg, 100 + gi)
+        Gk = rv.integers(-2, 3, size=(B, 2, 3)).astype(np.float64)
+        Ak = rv.uniform(0.5, 5.0, size=(B, 2))
+        vals = np.empty((V, B, 1))
+        for j in range(B):
+            vals[:, j, 0] = np.cos(2 * np.pi * rlu @ Gk[j].T) @ Ak[j]
+        vecs = rv.normal(size=(V, B, A, 3)) + 1j * rv.normal(size=(V, B, A, 3))
+        vecs /= np.linalg.norm(vecs.reshape(V, B, -1), axis=2)[:, :, None, None]
+        g.fill(vals, np.array([1, 0, 0]), vecs, np.array([0, 3 * A, 0, 2, 3]))
+        grids.append(g)
+        if verbose:
+            verbose(f"grid {gi}: sg {sg} A={A} B={B} V={V} (target {v_target:.0f})")
+    w = rng(cfg, 1).dirichlet(np.full(n_grids, 0.3))
+    counts = np.floor(w * n_q).astype(np.int64)
+    counts[np.argmax(counts)] += int(n_q) - int(counts.sum())
+    offsets = np.concatenate([[0], np.cumsum(counts)]).astype(np.uint64)
+    Q = rng(cfg, 2).uniform(-box, box, size=(int(n_q), 3))
+    return grids, Q, offsets

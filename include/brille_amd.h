@@ -184,7 +184,18 @@ typedef struct brille_perm_desc {
 
 enum brille_precision { BRILLE_FP64 = 0, BRILLE_FP32 = 1 };
 enum brille_memory { BRILLE_MEM_DEVICE = 0, BRILLE_MEM_HOST = 1 };
-enum brille_kernel { BRILLE_KERNEL_AUTO = 0, BRILLE_KERNEL_PLAIN = 1, BRILLE_KERNEL_TUNED = 2 };
+/* AUTO and TUNED pick TILE or BINNED by size; PLAIN is the literal FP64
+ * restatement (exact fold-in, atan2 antiphases); TILE is the fused one-launch
+ * kernel; BINNED buckets points by trellis node range first so the payload
+ * gathers hit in cache (device work space, not used when aux outputs are
+ * requested: those calls run TILE). */
+enum brille_kernel {
+  BRILLE_KERNEL_AUTO = 0,
+  BRILLE_KERNEL_PLAIN = 1,
+  BRILLE_KERNEL_TUNED = 2,
+  BRILLE_KERNEL_TILE = 3,
+  BRILLE_KERNEL_BINNED = 4
+};
 
 typedef struct brille_run_opts {
   int32_t no_move;       /* do_not_move_points: q is already irreducible */
@@ -237,17 +248,27 @@ int brille_ir_moveinto(brille_grid* grid, const double* Q, uint64_t n, int32_t m
                        uint32_t* invrot_out, uint8_t* status_out, void* stream);
 
 /* ---- many grids in one persistent launch (no reference counterpart) ---- */
+/* A batch holds filled grids (not owned; they must outlive it).  A run is one
+ * persistent kernel whose blocks take tiles of points from a work counter
+ * across all grids, captured once into a HIP graph per precision and schedule
+ * and replayed; the tile shape, block count, tiles per grab and grid order
+ * come from brille_batch_autotune (defaults before it runs). */
 int brille_batch_create(brille_grid* const* grids, uint32_t n_grids, brille_batch** out);
 /* Q: all points, grid g owns rows [offsets[g], offsets[g+1]); values_out[g] and
- * vectors_out[g] receive that grid's results.  Device memory only. */
+ * vectors_out[g] receive that grid's results.  Device memory only.  kernel =
+ * BRILLE_KERNEL_PLAIN or BRILLE_KERNEL_TILE run the grids one call after
+ * another instead (the unbatched baselines). */
 int brille_batch_run(brille_batch* batch, const double* Q, const uint64_t* offsets,
                      const brille_run_opts* opts, void* const* values_out,
                      void* const* vectors_out, void* stream);
+/* times every candidate schedule on these inputs and keeps the fastest */
 int brille_batch_autotune(brille_batch* batch, const double* Q, const uint64_t* offsets,
                           const brille_run_opts* opts, void* const* values_out,
                           void* const* vectors_out, void* stream);
-int brille_batch_schedule(const brille_batch* batch, uint32_t* points_per_lane,
-                          uint32_t* block_size, uint32_t* tile_points);
+/* the schedule in use: points per tile (= threads per block), persistent
+ * blocks, tiles per grab, grid order (0 as given, 1 costliest first) */
+int brille_batch_schedule(const brille_batch* batch, uint32_t* tile_points, uint32_t* blocks,
+                          uint32_t* tiles_per_grab, uint32_t* order);
 int brille_batch_destroy(brille_batch* batch);
 
 /* ---- learned tetrahedron proposal (no reference counterpart) ---- */
@@ -275,6 +296,11 @@ int brille_proposal_loss_grad(brille_grid* grid, const float* x, const int32_t* 
                               uint32_t n, float* loss, float* grads, uint32_t* n_params);
 int brille_proposal_get_params(const brille_grid* grid, float* params, uint32_t* n_params);
 int brille_proposal_set_params(brille_grid* grid, const float* params, uint32_t n_params);
+/* the network's proposal for points given by their features (n x 3 float, the
+ * scaled Cartesian coordinates) and any tetrahedron of their polyhedron node
+ * (in_tet selects the candidates); host memory */
+int brille_proposal_predict(brille_grid* grid, const float* x, const int32_t* in_tet, uint32_t n,
+                            int32_t* tet_out);
 
 /* ---- timing and device plumbing (used by the benchmark and tests) ---- */
 /* run the hot path `iters` times on `stream` and report per-kernel mean launch
