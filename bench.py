@@ -47,6 +47,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline (profiling runs)")
+    ap.add_argument("--no-learned", action="store_true", help="config 4: skip training the tetrahedron proposal")
+    ap.add_argument("--train-pairs", type=float, default=2e7, help="config 4: (q, tetrahedron) training pairs")
+    ap.add_argument("--train-epochs", type=int, default=3)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only copy this many MiB device-to-device once (calibrates the HBM byte counters) and exit")
     args = ap.parse_args()
@@ -107,24 +110,59 @@ def main():
         for _ in range(args.warmup):
             run(opts, 1)
         capi.check(lib.brille_device_sync())
-        kms = []
+        tot, parts = [], []
         w0 = time.perf_counter()
         for s in range(args.steps):
-            kms.append(run(opts, 1).ms[0])
+            t = run(opts, 1)
+            tot.append(t.total_ms)
+            parts.append([t.ms[k] for k in range(t.n_kernels)])
         capi.check(lib.brille_device_sync())
         wall = (time.perf_counter() - w0) * 1000.0 / args.steps
-        log(f"{label}: {wall:.3f} ms/step wall, kernel {np.mean(kms):.3f} ms, {n / wall * 1e3:.4g} Q/s")
-        return wall, float(np.mean(kms))
+        names = [t.names[k].value.decode() for k in range(t.n_kernels)]
+        stages = {nm: round(float(v), 4) for nm, v in zip(names, np.mean(parts, axis=0))}
+        log(f"{label}: {wall:.3f} ms/step wall, kernels {np.mean(tot):.3f} ms {stages}, {n / wall * 1e3:.4g} Q/s")
+        return dict(value=n / wall * 1e3, ms_per_step=wall, kernel_ms=float(np.mean(tot)), stages=stages)
 
-    fast = capi.RunOpts(0, capi.FP32, 0, capi.MEM_DEVICE, capi.KERNEL_TUNED, 0)
-    wall_fast, k_fast = timed(fast, "tuned fp32")
+    results = {}
+    learned = None
+    if cfg == 4 and not args.no_learned:
+        topts = capi.TrainOpts(int(args.train_pairs), 2026, args.train_epochs, 8192, 2e-3)
+        rep = capi.TrainReport()
+        capi.check(lib.brille_proposal_train(dev.h, C.byref(topts), C.byref(rep)))
+        learned = dict(pairs=int(rep.n_pairs), epochs=args.train_epochs, final_loss=float(rep.final_loss),
+                       held_out_hit_rate=float(rep.train_hit_rate), tetrahedra=int(rep.n_classes),
+                       train_ms=float(rep.ms))
+        log(f"proposal trained: {learned}")
+        results["learned+binned fp32"] = timed(capi.RunOpts(0, capi.FP32, 1, capi.MEM_DEVICE, capi.KERNEL_BINNED, 0),
+                                               "learned+binned fp32")
+        tried, hits = C.c_uint64(), C.c_uint64()
+        capi.check(lib.brille_proposal_stats(dev.h, C.byref(tried), C.byref(hits)))
+        learned["eval_proposals"] = int(tried.value)
+        learned["eval_verified"] = int(hits.value)
+        learned["eval_hit_rate"] = hits.value / max(1, tried.value)
+    results["binned fp32"] = timed(capi.RunOpts(0, capi.FP32, 0, capi.MEM_DEVICE, capi.KERNEL_BINNED, 0), "binned fp32")
+    results["tile fp32"] = timed(capi.RunOpts(0, capi.FP32, 0, capi.MEM_DEVICE, capi.KERNEL_TILE, 0), "tile fp32")
     plain_res = None
     if not args.no_plain:
-        plain = capi.RunOpts(0, capi.FP64, 0, capi.MEM_DEVICE, capi.KERNEL_PLAIN, 0)
-        wall_plain, k_plain = timed(plain, "plain fp64")
-        plain_res = dict(value=n / wall_plain * 1e3, ms_per_step=wall_plain, kernel_ms=k_plain,
-                         dtype="f64", bytes_per_q=24 + B * sv * 8 + B * sc * 16)
+        plain_res = timed(capi.RunOpts(0, capi.FP64, 0, capi.MEM_DEVICE, capi.KERNEL_PLAIN, 0), "plain fp64")
+        plain_res.update(dtype="f64", bytes_per_q=24 + B * sv * 8 + B * sc * 16)
     capi.check(lib.brille_stream_destroy(stream))
+    head_name = "learned+binned fp32" if "learned+binned fp32" in results else "binned fp32"
+    # FP32 error of the headline path against the FP64 path (itself at 1e-12 of
+    # the oracle) on the first 1e6 points of the workload
+    m = min(n, 1_000_000)
+    v32, c32 = dev.ir_interpolate_at(Q[:m], precision=capi.FP32, kernel=capi.KERNEL_BINNED,
+                                     use_proposal=learned is not None)
+    v64, c64 = dev.ir_interpolate_at(Q[:m], precision=capi.FP64, kernel=capi.KERNEL_TILE)
+    vscale = np.max(np.abs(v64), axis=0, keepdims=True)
+    cnorm = np.sqrt(np.sum(np.abs(c64.reshape(m, B, -1)) ** 2, axis=2, keepdims=True))
+    fp32_error = dict(sample=m, eigenvalue=float(np.max(np.abs(v32 - v64) / vscale)),
+                      eigenvector=float(np.max(np.abs(c32 - c64).reshape(m, B, -1) / cnorm)),
+                      measure="max |FP32 - FP64| over the mode's value scale / eigenvector norm")
+    log(f"fp32 error vs fp64: {fp32_error}")
+    head = results[head_name]
+    wall_fast = head["ms_per_step"]
+    k_fast = head["kernel_ms"]
 
     bytes_per_q = 24 + B * sv * 4 + B * sc * 8
     tb = table_bytes(T)
@@ -133,7 +171,8 @@ def main():
     per_q = measured_traffic(cfg)
     roofline = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
                     traffic=(per_q * n if per_q else None), bytes_per_q=bytes_per_q, table_bytes=tb,
-                    traffic_bytes_per_q=per_q, kernel="k_ir_interpolate<float, tuned, Gamma layout>")
+                    traffic_bytes_per_q=per_q, timed=f"{head_name}: all kernels of one call (HIP events)",
+                    kernel="k_bin_sums<float, Gamma layout> (largest stage)", stages=head["stages"])
     cpu = None if args.no_cpu else cpu_baseline(T, cfg, c, args.cpu_seconds)
     out = {
         "metric": "Q-points/s ir_interpolate_at, 1 MI355X: learned+batched vs plain, %HBM peak",
@@ -149,13 +188,17 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"config{cfg}: {c['name']}, node volume {c['volume_fraction']:g} x V_IR, "
-                               f"{n:.3g} Q in [-{c['box']},{c['box']}]^3, FP64 fold-in/weights, FP32/c64 sums",
+                               f"{n:.3g} Q in [-{c['box']},{c['box']}]^3, FP64 fold-in/weights, FP32/c64 sums; "
+                               f"headline path: {head_name}",
                    "n_q": n, "vertices": int(T["vertices"].shape[0]), "tetrahedra": int(T["tet_volume"].size),
                    "modes": B, "parallelism": "single GPU"},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "plain": plain_res,
         "speedup_vs_plain": (plain_res["ms_per_step"] / wall_fast) if plain_res else None,
+        "paths": results,
+        "learned": learned,
+        "fp32_error": fp32_error,
     }
     print(json.dumps(out), flush=True)
 
