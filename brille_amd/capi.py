@@ -358,7 +358,8 @@ class Grid:
         return vs, cs
 
     def ir_interpolate_at(self, Q, precision=FP64, kernel=KERNEL_AUTO, no_move=False, aux=False, use_proposal=False):
-        """Host-memory call; returns (values, vectors[, aux dict])."""
+        """Host-memory call; returns (values, vectors[, aux dict]).  aux: True for
+        every per-point output, or a dict naming outputs to leave out."""
         Q = np.ascontiguousarray(Q, dtype=np.float64).reshape(-1, 3)
         n = Q.shape[0]
         vs, cs = self.out_shapes(n)
@@ -372,6 +373,8 @@ class Grid:
                         invrot=np.zeros(n, np.uint32), node=np.zeros(n, np.uint32), tet=np.zeros(n, np.uint32),
                         n_iw=np.zeros(n, np.uint8), iw_index=np.zeros((n, 8), np.uint32),
                         iw_weight=np.zeros((n, 8)), status=np.zeros(n, np.uint8))
+            if isinstance(aux, dict):  # {name: False} leaves an output out (the binned pipeline has no tau)
+                bufs = {k: v for k, v in bufs.items() if aux.get(k, True)}
             a = AuxOut(**{k: v.ctypes.data_as(C.c_void_p) for k, v in bufs.items()})
         check(lib().brille_ir_interpolate_at(self.h, Q.ctypes.data_as(C.c_void_p), n, C.byref(o),
                                              vals.ctypes.data_as(C.c_void_p), vecs.ctypes.data_as(C.c_void_p),

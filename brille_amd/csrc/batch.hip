@@ -388,27 +388,33 @@ int brille_batch_autotune(brille_batch* b, const double* Q, const uint64_t* offs
   BATCH_TRY(hipEventCreate(&e1));
   Schedule best = b->sched;
   float best_ms = -1.f;
+  auto time_one = [&](const Schedule& sc, float* ms_out) -> int {
+    int r = run_with(b, Q, offsets, o, values_out, vectors_out, s, sc);  // warm (and capture)
+    if (r != BRILLE_OK) return r;
+    float tot = 0.f;
+    for (int rep = 0; rep < 2; ++rep) {
+      (void)hipEventRecord(e0, s);
+      r = run_with(b, Q, offsets, o, values_out, vectors_out, s, sc);
+      if (r != BRILLE_OK) return r;
+      (void)hipEventRecord(e1, s);
+      (void)hipEventSynchronize(e1);
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      tot += ms;
+    }
+    *ms_out = tot;
+    return BRILLE_OK;
+  };
   for (uint32_t tile : {128u, 256u})
     for (uint32_t bpc : {2u, 4u, 8u})
       for (uint32_t grab : {1u, 4u})
         for (uint32_t order : {0u, 1u}) {
+          if (rc != BRILLE_OK) break;
           Schedule sc{tile, bpc, grab, order};
-          rc = run_with(b, Q, offsets, o, values_out, vectors_out, s, sc);  // warm (and capture)
-          if (rc != BRILLE_OK) break;
-          float tot = 0.f;
-          for (int r = 0; r < 2; ++r) {
-            (void)hipEventRecord(e0, s);
-            rc = run_with(b, Q, offsets, o, values_out, vectors_out, s, sc);
-            if (rc != BRILLE_OK) break;
-            (void)hipEventRecord(e1, s);
-            (void)hipEventSynchronize(e1);
-            float ms = 0.f;
-            (void)hipEventElapsedTime(&ms, e0, e1);
-            tot += ms;
-          }
-          if (rc != BRILLE_OK) break;
-          if (best_ms < 0.f || tot < best_ms) {
-            best_ms = tot;
+          float ms = 0.f;
+          rc = time_one(sc, &ms);
+          if (rc == BRILLE_OK && (best_ms < 0.f || ms < best_ms)) {
+            best_ms = ms;
             best = sc;
           }
         }
