@@ -36,7 +36,8 @@ enum BatchKind : int32_t {
   BK_SCALAR1 = 3,
   BK_SCALAR2 = 4,
   BK_SCALAR3 = 5,
-  BK_SCALAR4 = 6
+  BK_SCALAR4 = 6,
+  BK_WIDE = 7
 };
 
 struct BatchGrid {
@@ -72,13 +73,14 @@ static int32_t batch_kind(const DataConst& d) {
   if (L == LAYOUT_GAMMA && d.sv == 1 && d.sc == 3) return BK_GAMMA3;
   if (L == LAYOUT_GAMMA && d.sv == 1 && d.sc == 6) return BK_GAMMA6;
   if (L == LAYOUT_SCALAR && d.sv == 1 && d.sc >= 1 && d.sc <= 4) return BK_SCALAR1 + (int32_t)d.sc - 1;
+  if (wide_rows(d)) return BK_WIDE;
   return BK_GENERIC;
 }
 
 template <typename Real, int L, int SC, int SV, int TB>
 __device__ __forceinline__ void batch_tile(const BatchGrid& bg, const BatchSlot& sl, uint64_t p0, uint32_t npts,
                                            PointRec<Real>* s_rec, typename Cplx<Real>::T* s_ph, double* s_q,
-                                           int no_move) {
+                                           typename Cplx<Real>::T* s_wide, int no_move) {
   const uint32_t tid = threadIdx.x;
   PointRec<Real>& r = s_rec[tid];
   r.nslot = 0;
@@ -92,7 +94,7 @@ __device__ __forceinline__ void batch_tile(const BatchGrid& bg, const BatchSlot&
   }
   __syncthreads();
   tile_sums<Real, true, L, SC, SV, TB>(bg.g, bg.d, s_rec, s_ph, s_q, nullptr, p0, npts, bg.ph_in_lds != 0, sl.vals,
-                                      sl.vecs);
+                                      sl.vecs, s_wide);
   __syncthreads();
 }
 
@@ -103,6 +105,7 @@ __global__ __launch_bounds__(TB) void k_batch(const BatchGrid* grids, const Batc
   __shared__ PointRec<Real> s_rec[TB];
   __shared__ C s_ph[TB * 4];
   __shared__ double s_q[3 * TB];
+  __shared__ C s_wide[(TB / 64) * kWideMax];
   __shared__ unsigned long long s_t;
   const uint64_t n_tiles = hdr->n_tiles;
   const uint32_t n_slots = hdr->n_slots;
@@ -128,13 +131,14 @@ __global__ __launch_bounds__(TB) void k_batch(const BatchGrid* grids, const Batc
       const uint64_t left = sl.n - p0;
       const uint32_t npts = left < (uint64_t)TB ? (uint32_t)left : (uint32_t)TB;
       switch (bg.kind) {
-        case BK_GAMMA3: batch_tile<Real, LAYOUT_GAMMA, 3, 1, TB>(bg, sl, p0, npts, s_rec, s_ph, s_q, no_move); break;
-        case BK_GAMMA6: batch_tile<Real, LAYOUT_GAMMA, 6, 1, TB>(bg, sl, p0, npts, s_rec, s_ph, s_q, no_move); break;
-        case BK_SCALAR1: batch_tile<Real, LAYOUT_SCALAR, 1, 1, TB>(bg, sl, p0, npts, s_rec, s_ph, s_q, no_move); break;
-        case BK_SCALAR2: batch_tile<Real, LAYOUT_SCALAR, 2, 1, TB>(bg, sl, p0, npts, s_rec, s_ph, s_q, no_move); break;
-        case BK_SCALAR3: batch_tile<Real, LAYOUT_SCALAR, 3, 1, TB>(bg, sl, p0, npts, s_rec, s_ph, s_q, no_move); break;
-        case BK_SCALAR4: batch_tile<Real, LAYOUT_SCALAR, 4, 1, TB>(bg, sl, p0, npts, s_rec, s_ph, s_q, no_move); break;
-        default: batch_tile<Real, LAYOUT_GENERIC, 0, 0, TB>(bg, sl, p0, npts, s_rec, s_ph, s_q, no_move); break;
+        case BK_GAMMA3: batch_tile<Real, LAYOUT_GAMMA, 3, 1, TB>(bg, sl, p0, npts, s_rec, s_ph, s_q, s_wide, no_move); break;
+        case BK_GAMMA6: batch_tile<Real, LAYOUT_GAMMA, 6, 1, TB>(bg, sl, p0, npts, s_rec, s_ph, s_q, s_wide, no_move); break;
+        case BK_SCALAR1: batch_tile<Real, LAYOUT_SCALAR, 1, 1, TB>(bg, sl, p0, npts, s_rec, s_ph, s_q, s_wide, no_move); break;
+        case BK_SCALAR2: batch_tile<Real, LAYOUT_SCALAR, 2, 1, TB>(bg, sl, p0, npts, s_rec, s_ph, s_q, s_wide, no_move); break;
+        case BK_SCALAR3: batch_tile<Real, LAYOUT_SCALAR, 3, 1, TB>(bg, sl, p0, npts, s_rec, s_ph, s_q, s_wide, no_move); break;
+        case BK_SCALAR4: batch_tile<Real, LAYOUT_SCALAR, 4, 1, TB>(bg, sl, p0, npts, s_rec, s_ph, s_q, s_wide, no_move); break;
+        case BK_WIDE: batch_tile<Real, LAYOUT_WIDE, 0, 0, TB>(bg, sl, p0, npts, s_rec, s_ph, s_q, s_wide, no_move); break;
+        default: batch_tile<Real, LAYOUT_GENERIC, 0, 0, TB>(bg, sl, p0, npts, s_rec, s_ph, s_q, s_wide, no_move); break;
       }
     }
   }

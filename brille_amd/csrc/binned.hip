@@ -202,6 +202,7 @@ __global__ __launch_bounds__(TILE) void k_bin_sums(GridConst g, DataConst d, Pro
   __shared__ C s_ph[TILE * S::NPH];
   __shared__ double s_q[S::NQ];
   __shared__ uint32_t s_out[TILE];
+  __shared__ C s_wide[S::NW];
   __shared__ uint32_t s_tried, s_verified;
   if (Prop && threadIdx.x == 0) {
     s_tried = 0;
@@ -255,7 +256,7 @@ __global__ __launch_bounds__(TILE) void k_bin_sums(GridConst g, DataConst d, Pro
     atomicAdd(&pc.stats[1], (unsigned long long)s_verified);
   }
   const uint32_t npts = n - tile0 < (uint32_t)TILE ? n - tile0 : (uint32_t)TILE;
-  tile_sums<Real, true, L, SC, SV>(g, d, s_rec, s_ph, s_q, s_out, 0, npts, ph_in_lds, vals, vecs);
+  tile_sums<Real, true, L, SC, SV>(g, d, s_rec, s_ph, s_q, s_out, 0, npts, ph_in_lds, vals, vecs, s_wide);
 }
 
 using SumsKernel = void (*)(GridConst, DataConst, ProposalConst, const SortRec*, uint32_t, void*, void*,
@@ -287,6 +288,7 @@ static SumsKernel pick_sums(const DataConst& d, bool prop) {
       default: break;
     }
   }
+  if (wide_rows(d)) return k_bin_sums<Real, LAYOUT_WIDE, 0, 0, false>;
   return k_bin_sums<Real, LAYOUT_GENERIC, 0, 0, false>;
 }
 

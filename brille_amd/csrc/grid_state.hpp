@@ -48,6 +48,7 @@ void free_proposal(brille_grid* G);
 void free_staging(brille_grid* G);
 int check_rotation_modes(const brille_grid* G);
 int payload_layout(const DataConst& d);
+bool wide_rows(const DataConst& d);
 // binned pipeline (binned.hip)
 bool binned_applies(const brille_grid* G, const LaunchArgs& a, int kernel);
 int launch_binned(brille_grid* G, const LaunchArgs& a, int precision, hipStream_t s, float* stage_ms);

@@ -173,6 +173,33 @@ __device__ __forceinline__ typename Cplx<Real>::T antiphase_of(Real re, Real im)
   return e;
 }
 
+// The closed form without branches, for the unrolled slot loops: re and im
+// are rescaled by the power of two that brings the larger to [0.5, 1) (exact,
+// and the result is the unscaled formula's wherever that one neither
+// underflows nor overflows), so |z|^2 is in [0.25, 2] for the reciprocal
+// square root; z real and positive gives exactly 1, z = 0 gives +-1 by the
+// sign of re, as atan2 does.
+template <typename Real>
+__device__ __forceinline__ typename Cplx<Real>::T antiphase_closed(Real re, Real im) {
+  typename Cplx<Real>::T e;
+  const Real m = fmax(fabs(re), fabs(im));
+  int k;
+  (void)frexp(m, &k);
+  const Real a = ldexp(re, -k), b = ldexp(im, -k);
+  const Real r2 = a * a + b * b;
+  Real inv;
+  if constexpr (sizeof(Real) == 4) inv = __builtin_amdgcn_rsqf(r2);  // v_rsq_f32, 1 ulp
+  else inv = 1.0 / sqrt(r2);
+  Real ex = a * inv, ey = -b * inv;
+  const bool zero = m == (Real)0;
+  const bool pos_real = im == (Real)0 && re > (Real)0;
+  ex = pos_real ? (Real)1 : (zero ? (signbit(re) ? (Real)-1 : (Real)1) : ex);
+  ey = (pos_real || zero) ? (Real)0 : ey;
+  e.x = ex;
+  e.y = ey;
+  return e;
+}
+
 // Hermitian product sum_k conj(p_k) x_k over n complex elements, in the
 // reference's order (re += p.re x.re + p.im x.im; im += p.re x.im - p.im x.re)
 template <typename Real>
@@ -480,28 +507,35 @@ __device__ __forceinline__ void unit_onepass(const GridConst& g, const DataConst
   for (int k = 0; k < 2 * SC; ++k) acc[k] = 0;
 #pragma unroll
   for (int s = 0; s < SV; ++s) vacc[s] = 0;
-  const int nslot = r.nslot;
+  // The rows of a group of slots are all requested before any is used, so a
+  // wave keeps GR rows per lane in flight (8 in FP32, 4 in FP64 for the
+  // register budget).  Unused tetrahedron slots repeat the pivot with weight 0:
+  // their loads hit in cache and add exact zeros, and no branch splits the loads.
+  constexpr int GR = sizeof(Real) == 4 ? 8 : 4;
 #pragma unroll
-  for (int x = 0; x < 8; ++x) {
-    if (x == 4 && nslot <= 4) break;
-    const Real w = r.w[x];
-    Real row[NR];
-    load_row<Real, NR>(row, base + ((uint64_t)r.idx[x] * d.B + b) * stride);
-    Real re = 0, im = 0;
+  for (int x0 = 0; x0 < 8; x0 += GR) {
+    Real row[GR][NR];
 #pragma unroll
-    for (int k = 0; k < SC; ++k) {
-      re += pr[2 * k] * row[2 * k] + pr[2 * k + 1] * row[2 * k + 1];
-      im += pr[2 * k] * row[2 * k + 1] - pr[2 * k + 1] * row[2 * k];
+    for (int x = 0; x < GR; ++x) load_row<Real, NR>(row[x], base + ((uint64_t)r.idx[x0 + x] * d.B + b) * stride);
+#pragma unroll
+    for (int x = 0; x < GR; ++x) {
+      const Real w = r.w[x0 + x];
+      Real re = 0, im = 0;
+#pragma unroll
+      for (int k = 0; k < SC; ++k) {
+        re += pr[2 * k] * row[x][2 * k] + pr[2 * k + 1] * row[x][2 * k + 1];
+        im += pr[2 * k] * row[x][2 * k + 1] - pr[2 * k + 1] * row[x][2 * k];
+      }
+      const C e = antiphase_closed<Real>(re, im);
+      const Real cx = w * e.x, cy = w * e.y;
+#pragma unroll
+      for (int k = 0; k < SC; ++k) {
+        acc[2 * k] += cx * row[x][2 * k] - cy * row[x][2 * k + 1];
+        acc[2 * k + 1] += cx * row[x][2 * k + 1] + cy * row[x][2 * k];
+      }
+#pragma unroll
+      for (int s = 0; s < SV; ++s) vacc[s] += w * row[x][2 * SC + s];
     }
-    const C e = antiphase_of<Real, false>(re, im);
-    const Real cx = w * e.x, cy = w * e.y;
-#pragma unroll
-    for (int k = 0; k < SC; ++k) {
-      acc[2 * k] += cx * row[2 * k] - cy * row[2 * k + 1];
-      acc[2 * k + 1] += cx * row[2 * k + 1] + cy * row[2 * k];
-    }
-#pragma unroll
-    for (int s = 0; s < SV; ++s) vacc[s] += w * row[2 * SC + s];
   }
 #pragma unroll
   for (int s = 0; s < SV; ++s) ov[s] = vacc[s];
@@ -534,8 +568,98 @@ __device__ __forceinline__ void unit_onepass(const GridConst& g, const DataConst
 enum Layout : int {
   LAYOUT_GENERIC = 0,  // any elements, rotations and permutation table
   LAYOUT_GAMMA = 1,    // scalar values; vectors = one 3-vector per atom rotating like Gamma; no permutations
-  LAYOUT_SCALAR = 2    // scalar values and scalar vectors; no permutations
+  LAYOUT_SCALAR = 2,   // scalar values and scalar vectors; no permutations
+  LAYOUT_WIDE = 3      // as GAMMA or SCALAR with long rows (up to kWideMax complex): one wave per (point, mode)
 };
+constexpr uint32_t kWideMax = 128;  // complex elements per row the wide phase 2 handles (two per lane)
+
+__device__ __forceinline__ float wave_sum_f(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+__device__ __forceinline__ double wave_sum_f(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+// Phase 2 for one (point, mode) unit by a whole wave, for rows too long for
+// one lane: lane l takes the row's complex elements l and l + 64, so every
+// row is read with coalesced loads; the Hermitian product is a wave sum.  The
+// Gamma rotation mixes the three elements of an atom, so the accumulated row
+// goes through the wave's LDS slice `lds` (kWideMax elements).
+template <typename Real>
+__device__ inline void unit_wide(const GridConst& g, const DataConst& d, const PointRec<Real>& r, const double* q,
+                                 uint32_t b, Real* ov, typename Cplx<Real>::T* oc, typename Cplx<Real>::T* lds,
+                                 bool gamma, bool cart) {
+  using C = typename Cplx<Real>::T;
+  const uint32_t l = threadIdx.x & 63u;
+  const uint32_t sc = d.sc, sv = d.sv;
+  const bool h0 = l < sc, h1 = l + 64u < sc;
+  const C zero = Elem<C>::zero();
+  const C* prow = reinterpret_cast<const C*>(row_at<Real>(d, r.pivot, b));
+  const C p0 = h0 ? prow[l] : zero, p1 = h1 ? prow[l + 64] : zero;
+  C a0 = zero, a1 = zero;
+  Real va = 0;
+  const int nslot = r.nslot;
+#pragma unroll
+  for (int x = 0; x < 8; ++x) {
+    if (x >= nslot) break;  // wave-uniform: one point per wave
+    const Real w = r.w[x];
+    const Real* row = row_at<Real>(d, r.idx[x], b);
+    const C* rc = reinterpret_cast<const C*>(row);
+    const C v0 = h0 ? rc[l] : zero, v1 = h1 ? rc[l + 64] : zero;
+    Real re = p0.x * v0.x + p0.y * v0.y;
+    re += p1.x * v1.x + p1.y * v1.y;
+    Real im = p0.x * v0.y - p0.y * v0.x;
+    im += p1.x * v1.y - p1.y * v1.x;
+    re = wave_sum_f(re);
+    im = wave_sum_f(im);
+    const C e = antiphase_closed<Real>(re, im);
+    const Real cx = w * e.x, cy = w * e.y;
+    a0.x += cx * v0.x - cy * v0.y;
+    a0.y += cx * v0.y + cy * v0.x;
+    a1.x += cx * v1.x - cy * v1.y;
+    a1.y += cx * v1.y + cy * v1.x;
+    if (l < sv) va += w * row[2 * sc + l];
+  }
+  if (l < sv) ov[l] = va;
+  if (!gamma) {
+    if (h0) oc[l] = a0;
+    if (h1) oc[l + 64] = a1;
+    return;
+  }
+  // rip_gamma_complex  src/interpolator_gamma.tpp:100-114, element by element
+  if (h0) lds[l] = a0;
+  if (h1) lds[l + 64] = a1;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const uint32_t el = l + 64u * h;
+    if (el < sc) {
+      const uint32_t a = el / 3u, i = el - 3u * a;
+      const C v[3] = {lds[3 * a], lds[3 * a + 1], lds[3 * a + 2]};
+      C t = zero;
+      if (cart) {
+        const double* R = d.cart_rot + 9 * (uint64_t)r.invrot + 3 * i;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) t = Elem<C>::add(t, Elem<C>::scale(v[j], (Real)R[j]));
+      } else {
+        const int32_t* R = g.rot + 9 * (uint64_t)r.invrot + 3 * i;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) t = Elem<C>::add(t, Elem<C>::scale(v[j], (Real)R[j]));
+      }
+      const C ph = gamma_phase<Real>(d, q, r.invrot, a);
+      oc[3u * d.F0[a * d.gamma_ops + r.invrot] + i] = cmul(ph, t);
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 struct LaunchArgs {
   const double* Q;
@@ -673,7 +797,8 @@ __device__ inline void write_aux(const brille_aux_out& aux, uint64_t p, const Lo
 template <int L, int SC, int TB = TILE>
 struct TileShape {
   static constexpr int NPH = L == LAYOUT_GAMMA ? SC / 3 : (L == LAYOUT_GENERIC ? 4 : 1);  // phases per point
-  static constexpr int NQ = L == LAYOUT_GENERIC ? 3 * TB : 1;                              // q kept in LDS
+  static constexpr int NQ = (L == LAYOUT_GENERIC || L == LAYOUT_WIDE) ? 3 * TB : 1;      // q kept in LDS
+  static constexpr int NW = L == LAYOUT_WIDE ? (TB / 64) * (int)kWideMax : 1;              // wide-row slices
 };
 
 // a located point's phase-2 record (plus phases, and q for the general layout)
@@ -698,6 +823,8 @@ __device__ __forceinline__ void store_record(const DataConst& d, const Located& 
     s_q[3 * slot] = P.q[0]; s_q[3 * slot + 1] = P.q[1]; s_q[3 * slot + 2] = P.q[2];
     if (ph_in_lds)
       for (uint32_t k = 0; k < n_atoms; ++k) s_ph[NPH * slot + k] = gamma_phase<Real>(d, P.q, P.invrot, k);
+  } else if constexpr (L == LAYOUT_WIDE) {
+    s_q[3 * slot] = P.q[0]; s_q[3 * slot + 1] = P.q[1]; s_q[3 * slot + 2] = P.q[2];
   }
 }
 
@@ -707,7 +834,7 @@ template <typename Real, bool Fast, int L, int SC, int SV, int TB = TILE>
 __device__ __forceinline__ void tile_sums(const GridConst& g, const DataConst& d, const PointRec<Real>* s_rec,
                                           const typename Cplx<Real>::T* s_ph, const double* s_q,
                                           const uint32_t* out_row, uint64_t row0, uint32_t npts, bool ph_in_lds,
-                                          void* vals_, void* vecs_) {
+                                          void* vals_, void* vecs_, typename Cplx<Real>::T* s_wide = nullptr) {
   using C = typename Cplx<Real>::T;
   constexpr int NPH = TileShape<L, SC>::NPH;
   const uint32_t B = d.B;
@@ -715,6 +842,26 @@ __device__ __forceinline__ void tile_sums(const GridConst& g, const DataConst& d
   Real* vals = (Real*)vals_;
   C* vecs = (C*)vecs_;
   const bool cart = d.vec_lu == BRILLE_LU_ANGSTROM;
+  if constexpr (L == LAYOUT_WIDE) {
+    // one wave per unit; wave-uniform loop
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    const bool gamma = d.vec_rot == BRILLE_ROT_GAMMA && d.ec[1] > 0;
+    C* lds = s_wide + wave * kWideMax;
+    for (uint32_t u = wave; u < units; u += TB / 64) {
+      const uint32_t pl = u / B, b = u - pl * B;
+      const PointRec<Real>& r = s_rec[pl];
+      const uint64_t row = ((out_row ? (uint64_t)out_row[pl] : row0 + pl)) * B + b;
+      Real* ov = vals + row * d.sv;
+      C* oc = vecs + row * d.sc;
+      if (r.nslot == 0) {
+        for (uint32_t e = lane; e < d.sv; e += 64) ov[e] = 0;
+        for (uint32_t e = lane; e < d.sc; e += 64) oc[e] = Elem<C>::zero();
+        continue;
+      }
+      unit_wide<Real>(g, d, r, s_q + 3 * pl, b, ov, oc, lds, gamma, cart);
+    }
+    return;
+  }
   for (uint32_t u = threadIdx.x; u < units; u += TB) {
     const uint32_t pl = u / B, b = u - pl * B;
     const PointRec<Real>& r = s_rec[pl];
@@ -754,6 +901,7 @@ __global__ __launch_bounds__(TILE) void k_ir_interpolate(GridConst g, DataConst 
   __shared__ PointRec<Real> s_rec[TILE];
   __shared__ C s_ph[TILE * S::NPH];
   __shared__ double s_q[S::NQ];
+  __shared__ C s_wide[S::NW];
   const uint64_t tile0 = (uint64_t)blockIdx.x * TILE;
   const uint32_t tid = threadIdx.x;
   const uint64_t p = tile0 + tid;
@@ -780,7 +928,7 @@ __global__ __launch_bounds__(TILE) void k_ir_interpolate(GridConst g, DataConst 
   __syncthreads();
   const uint64_t left = a.n - tile0;
   const uint32_t npts = left < (uint64_t)TILE ? (uint32_t)left : (uint32_t)TILE;
-  tile_sums<Real, Fast, L, SC, SV>(g, d, s_rec, s_ph, s_q, nullptr, tile0, npts, ph_in_lds, a.vals, a.vecs);
+  tile_sums<Real, Fast, L, SC, SV>(g, d, s_rec, s_ph, s_q, nullptr, tile0, npts, ph_in_lds, a.vals, a.vecs, s_wide);
 }
 
 }  // namespace brille_amd

@@ -87,6 +87,13 @@ int payload_layout(const DataConst& d) {
   return LAYOUT_GENERIC;
 }
 
+// Gamma or scalar payloads whose rows are too long for one lane (more than 12
+// and at most kWideMax complex elements, at most 64 values): one wave per row
+bool wide_rows(const DataConst& d) {
+  const int L = payload_layout(d);
+  return (L == LAYOUT_GAMMA || L == LAYOUT_SCALAR) && d.sc > 12 && d.sc <= kWideMax && d.sv <= 64;
+}
+
 // The instantiated specialisations: Gamma payloads of 1-4 atoms and scalar
 // payloads of 1-4 complex elements, each with one value per mode; anything
 // else runs the general phase 2.
@@ -111,6 +118,7 @@ static TileKernel pick_tuned(const DataConst& d) {
       default: break;
     }
   }
+  if (wide_rows(d)) return k_ir_interpolate<Real, true, LAYOUT_WIDE, 0, 0>;
   return k_ir_interpolate<Real, true, LAYOUT_GENERIC, 0, 0>;
 }
 
