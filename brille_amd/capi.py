@@ -105,6 +105,8 @@ SIGNATURES = {
     "brille_bytes_per_point": (C.c_uint64, [C.c_void_p, C.c_int32]),
     "brille_ir_interpolate_at": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(RunOpts), C.c_void_p,
                                            C.c_void_p, C.POINTER(AuxOut), C.c_void_p]),
+    "brille_interpolate_at": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(RunOpts), C.c_void_p,
+                                        C.c_void_p, C.POINTER(AuxOut), C.c_void_p]),
     "brille_ir_moveinto": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_int32, C.c_void_p, C.c_void_p,
                                      C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "brille_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_uint32, C.POINTER(C.c_void_p)]),
@@ -357,7 +359,12 @@ class Grid:
         cs = (n,) + self.data.vecs.shape[1:]
         return vs, cs
 
-    def ir_interpolate_at(self, Q, precision=FP64, kernel=KERNEL_AUTO, no_move=False, aux=False, use_proposal=False):
+    def interpolate_at(self, Q, precision=FP64, kernel=KERNEL_AUTO, no_move=False, aux=False):
+        """The first-zone variant (brille_interpolate_at); same returns as ir_interpolate_at."""
+        return self.ir_interpolate_at(Q, precision, kernel, no_move, aux, False, _entry="brille_interpolate_at")
+
+    def ir_interpolate_at(self, Q, precision=FP64, kernel=KERNEL_AUTO, no_move=False, aux=False, use_proposal=False,
+                          _entry="brille_ir_interpolate_at"):
         """Host-memory call; returns (values, vectors[, aux dict]).  aux: True for
         every per-point output, or a dict naming outputs to leave out."""
         Q = np.ascontiguousarray(Q, dtype=np.float64).reshape(-1, 3)
@@ -376,9 +383,9 @@ class Grid:
             if isinstance(aux, dict):  # {name: False} leaves an output out (the binned pipeline has no tau)
                 bufs = {k: v for k, v in bufs.items() if aux.get(k, True)}
             a = AuxOut(**{k: v.ctypes.data_as(C.c_void_p) for k, v in bufs.items()})
-        check(lib().brille_ir_interpolate_at(self.h, Q.ctypes.data_as(C.c_void_p), n, C.byref(o),
-                                             vals.ctypes.data_as(C.c_void_p), vecs.ctypes.data_as(C.c_void_p),
-                                             C.byref(a) if a is not None else None, None))
+        check(getattr(lib(), _entry)(self.h, Q.ctypes.data_as(C.c_void_p), n, C.byref(o),
+                                     vals.ctypes.data_as(C.c_void_p), vecs.ctypes.data_as(C.c_void_p),
+                                     C.byref(a) if a is not None else None, None))
         if aux:
             return vals, vecs, bufs
         return vals, vecs

@@ -64,7 +64,7 @@ struct BatchHeader {
   unsigned long long counter;
   uint64_t n_tiles;
   uint32_t n_slots;
-  int32_t no_move;
+  int32_t fold;  // Fold
 };
 
 // the specialised phase 2 a grid's payload takes inside the batch kernel
@@ -80,7 +80,7 @@ static int32_t batch_kind(const DataConst& d) {
 template <typename Real, int L, int SC, int SV, int TB>
 __device__ __forceinline__ void batch_tile(const BatchGrid& bg, const BatchSlot& sl, uint64_t p0, uint32_t npts,
                                            PointRec<Real>* s_rec, typename Cplx<Real>::T* s_ph, double* s_q,
-                                           typename Cplx<Real>::T* s_wide, int no_move) {
+                                           typename Cplx<Real>::T* s_wide, int fold) {
   const uint32_t tid = threadIdx.x;
   PointRec<Real>& r = s_rec[tid];
   r.nslot = 0;
@@ -88,13 +88,14 @@ __device__ __forceinline__ void batch_tile(const BatchGrid& bg, const BatchSlot&
     const uint64_t p = p0 + tid;
     double Qp[3] = {sl.Q[3 * p], sl.Q[3 * p + 1], sl.Q[3 * p + 2]};
     Located P;
-    locate<true>(bg.g, Qp, no_move, P);
+    locate<true>(bg.g, Qp, fold, P);
     if (P.status != BRILLE_PT_OK) report_point(bg.err, p, P.status);
     else store_record<Real, L, SC>(bg.d, P, r, s_ph, s_q, tid, bg.ph_in_lds != 0, bg.n_atoms);
   }
   __syncthreads();
-  tile_sums<Real, true, L, SC, SV, TB>(bg.g, bg.d, s_rec, s_ph, s_q, nullptr, p0, npts, bg.ph_in_lds != 0, sl.vals,
-                                      sl.vecs, s_wide);
+  // four rows in flight: the persistent kernel's registers are the largest of all its branches
+  tile_sums<Real, true, L, SC, SV, TB, 4>(bg.g, bg.d, s_rec, s_ph, s_q, nullptr, p0, npts, bg.ph_in_lds != 0, sl.vals,
+                                         sl.vecs, s_wide);
   __syncthreads();
 }
 
@@ -109,7 +110,7 @@ __global__ __launch_bounds__(TB) void k_batch(const BatchGrid* grids, const Batc
   __shared__ unsigned long long s_t;
   const uint64_t n_tiles = hdr->n_tiles;
   const uint32_t n_slots = hdr->n_slots;
-  const int no_move = hdr->no_move;
+  const int fold = hdr->fold;
   for (;;) {
     if (threadIdx.x == 0) s_t = atomicAdd(&hdr->counter, (unsigned long long)grab);
     __syncthreads();
@@ -131,14 +132,14 @@ __global__ __launch_bounds__(TB) void k_batch(const BatchGrid* grids, const Batc
       const uint64_t left = sl.n - p0;
       const uint32_t npts = left < (uint64_t)TB ? (uint32_t)left : (uint32_t)TB;
       switch (bg.kind) {
-        case BK_GAMMA3: batch_tile<Real, LAYOUT_GAMMA, 3, 1, TB>(bg, sl, p0, npts, s_rec, s_ph, s_q, s_wide, no_move); break;
-        case BK_GAMMA6: batch_tile<Real, LAYOUT_GAMMA, 6, 1, TB>(bg, sl, p0, npts, s_rec, s_ph, s_q, s_wide, no_move); break;
-        case BK_SCALAR1: batch_tile<Real, LAYOUT_SCALAR, 1, 1, TB>(bg, sl, p0, npts, s_rec, s_ph, s_q, s_wide, no_move); break;
-        case BK_SCALAR2: batch_tile<Real, LAYOUT_SCALAR, 2, 1, TB>(bg, sl, p0, npts, s_rec, s_ph, s_q, s_wide, no_move); break;
-        case BK_SCALAR3: batch_tile<Real, LAYOUT_SCALAR, 3, 1, TB>(bg, sl, p0, npts, s_rec, s_ph, s_q, s_wide, no_move); break;
-        case BK_SCALAR4: batch_tile<Real, LAYOUT_SCALAR, 4, 1, TB>(bg, sl, p0, npts, s_rec, s_ph, s_q, s_wide, no_move); break;
-        case BK_WIDE: batch_tile<Real, LAYOUT_WIDE, 0, 0, TB>(bg, sl, p0, npts, s_rec, s_ph, s_q, s_wide, no_move); break;
-        default: batch_tile<Real, LAYOUT_GENERIC, 0, 0, TB>(bg, sl, p0, npts, s_rec, s_ph, s_q, s_wide, no_move); break;
+        case BK_GAMMA3: batch_tile<Real, LAYOUT_GAMMA, 3, 1, TB>(bg, sl, p0, npts, s_rec, s_ph, s_q, s_wide, fold); break;
+        case BK_GAMMA6: batch_tile<Real, LAYOUT_GAMMA, 6, 1, TB>(bg, sl, p0, npts, s_rec, s_ph, s_q, s_wide, fold); break;
+        case BK_SCALAR1: batch_tile<Real, LAYOUT_SCALAR, 1, 1, TB>(bg, sl, p0, npts, s_rec, s_ph, s_q, s_wide, fold); break;
+        case BK_SCALAR2: batch_tile<Real, LAYOUT_SCALAR, 2, 1, TB>(bg, sl, p0, npts, s_rec, s_ph, s_q, s_wide, fold); break;
+        case BK_SCALAR3: batch_tile<Real, LAYOUT_SCALAR, 3, 1, TB>(bg, sl, p0, npts, s_rec, s_ph, s_q, s_wide, fold); break;
+        case BK_SCALAR4: batch_tile<Real, LAYOUT_SCALAR, 4, 1, TB>(bg, sl, p0, npts, s_rec, s_ph, s_q, s_wide, fold); break;
+        case BK_WIDE: batch_tile<Real, LAYOUT_WIDE, 0, 0, TB>(bg, sl, p0, npts, s_rec, s_ph, s_q, s_wide, fold); break;
+        default: batch_tile<Real, LAYOUT_GENERIC, 0, 0, TB>(bg, sl, p0, npts, s_rec, s_ph, s_q, s_wide, fold); break;
       }
     }
   }
@@ -224,7 +225,7 @@ static double point_cost(const brille_grid* G) {
 
 // slots of one run (empty grids take no slot) and their upload
 static int stage_slots(brille_batch* b, const double* Q, const uint64_t* offsets, void* const* vals,
-                       void* const* vecs, const Schedule& sc, int no_move, hipStream_t s) {
+                       void* const* vecs, const Schedule& sc, int fold, hipStream_t s) {
   const uint32_t ng = (uint32_t)b->grids.size();
   std::vector<uint32_t> order(ng);
   for (uint32_t g = 0; g < ng; ++g) order[g] = g;
@@ -250,7 +251,7 @@ static int stage_slots(brille_batch* b, const double* Q, const uint64_t* offsets
   h.counter = 0;
   h.n_tiles = tiles;
   h.n_slots = (uint32_t)slots.size();
-  h.no_move = no_move;
+  h.fold = fold;
   if (!slots.empty())
     BATCH_TRY(hipMemcpyAsync(b->d_slots, slots.data(), slots.size() * sizeof(BatchSlot), hipMemcpyHostToDevice, s));
   BATCH_TRY(hipMemcpyAsync(b->d_hdr, &h, sizeof(h), hipMemcpyHostToDevice, s));
@@ -304,7 +305,7 @@ static int run_sequential(brille_batch* b, const double* Q, const uint64_t* offs
 
 static int run_with(brille_batch* b, const double* Q, const uint64_t* offsets, const brille_run_opts& o,
                     void* const* vals, void* const* vecs, hipStream_t s, const Schedule& sc) {
-  int rc = stage_slots(b, Q, offsets, vals, vecs, sc, o.no_move, s);
+  int rc = stage_slots(b, Q, offsets, vals, vecs, sc, o.no_move ? FOLD_NONE : FOLD_IR, s);
   if (rc != BRILLE_OK) return rc;
   hipGraphExec_t exec;
   rc = graph_for(b, o.precision, sc, &exec);

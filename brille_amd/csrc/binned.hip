@@ -47,7 +47,7 @@ __device__ __forceinline__ uint32_t bucket_of(uint32_t node, uint32_t meta, uint
 }
 
 template <bool Fast>
-__global__ __launch_bounds__(BIN_BLOCK) void k_bin_fold(GridConst g, const double* Q, uint32_t n, int no_move,
+__global__ __launch_bounds__(BIN_BLOCK) void k_bin_fold(GridConst g, const double* Q, uint32_t n, int fold,
                                                         FoldRec* rec, uint32_t* hist, uint32_t nblk, uint32_t nb,
                                                         ErrRec* err, uint64_t base) {
   extern __shared__ uint32_t h[];  // nb + 1 counters
@@ -59,7 +59,7 @@ __global__ __launch_bounds__(BIN_BLOCK) void k_bin_fold(GridConst g, const doubl
     if (p < n) {
       double Qp[3] = {Q[3 * (uint64_t)p], Q[3 * (uint64_t)p + 1], Q[3 * (uint64_t)p + 2]};
       Located L;
-      fold_node<Fast>(g, Qp, no_move, L);
+      fold_node<Fast>(g, Qp, fold, L);
       if (L.status != BRILLE_PT_OK) report_point(err, base + p, L.status);
       FoldRec r;
       r.q[0] = L.q[0]; r.q[1] = L.q[1]; r.q[2] = L.q[2];
@@ -375,7 +375,7 @@ int launch_binned(brille_grid* G, const LaunchArgs& a, int precision, hipStream_
     const uint32_t np = (uint32_t)((M + SCAN_SEG - 1) / SCAN_SEG);
     const size_t lds1 = (size_t)(nb + 1) * 4;
     if (evp) (void)hipEventRecord(evp[0], s);
-    hipLaunchKernelGGL(k_bin_fold<true>, dim3(nblk), dim3(BIN_BLOCK), lds1, s, G->g, a.Q + 3 * c0, m, a.no_move, rec,
+    hipLaunchKernelGGL(k_bin_fold<true>, dim3(nblk), dim3(BIN_BLOCK), lds1, s, G->g, a.Q + 3 * c0, m, a.fold, rec,
                        hist, nblk, nb, a.err, a.base + c0);
     if (evp) (void)hipEventRecord(evp[1], s);
     hipLaunchKernelGGL(k_scan_reduce, dim3(np), dim3(BIN_BLOCK), 0, s, hist, M, part);

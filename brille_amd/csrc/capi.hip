@@ -373,6 +373,12 @@ int brille_ir_interpolate_at(brille_grid* G, const double* Q, uint64_t n, const 
   return run_interpolate(G, Q, n, opts_in, values_out, vectors_out, aux, static_cast<hipStream_t>(stream), nullptr, 0);
 }
 
+int brille_interpolate_at(brille_grid* G, const double* Q, uint64_t n, const brille_run_opts* opts_in,
+                          void* values_out, void* vectors_out, const brille_aux_out* aux, void* stream) {
+  return run_interpolate(G, Q, n, opts_in, values_out, vectors_out, aux, static_cast<hipStream_t>(stream), nullptr, 0,
+                         FOLD_FIRST_ZONE);
+}
+
 int brille_ir_moveinto(brille_grid* G, const double* Q, uint64_t n, int32_t memory, double* q_out, int32_t* tau_out,
                        uint32_t* rot_out, uint32_t* invrot_out, uint8_t* status_out, void* stream_v) {
   if (!G || (!Q && n)) return fail(BRILLE_ERR_INVALID_ARGUMENT, "null argument");

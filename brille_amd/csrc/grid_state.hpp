@@ -55,7 +55,8 @@ int launch_binned(brille_grid* G, const LaunchArgs& a, int precision, hipStream_
 void free_binned(brille_grid* G);
 // the first failing point of a call as the reference's error text
 int report_point_error(const ErrRec& e, const double* Q, uint64_t n, bool host_mem);
+// fold: FOLD_IR (ir_interpolate_at) or FOLD_FIRST_ZONE (interpolate_at); opts->no_move wins
 int run_interpolate(brille_grid* G, const double* Q, uint64_t n, const brille_run_opts* opts, void* values_out,
                     void* vectors_out, const brille_aux_out* aux, hipStream_t s, brille_kernel_times* times,
-                    uint32_t iters);
+                    uint32_t iters, int fold = FOLD_IR);
 }  // namespace brille_amd

@@ -316,7 +316,7 @@ uint64_t BZTrellisQdc::bytes_per_point(int precision) const {
 }
 
 void BZTrellisQdc::ir_interpolate_at(const double* Q, uint64_t n, bool no_move, int precision, int kernel, void* vals,
-                                     void* vecs, const brille_aux_out* aux) {
+                                     void* vecs, const brille_aux_out* aux, bool first_zone) {
   brille_grid* h = handle();
   brille_run_opts o{};
   o.no_move = no_move ? 1 : 0;
@@ -324,7 +324,8 @@ void BZTrellisQdc::ir_interpolate_at(const double* Q, uint64_t n, bool no_move, 
   o.memory = BRILLE_MEM_HOST;
   o.kernel = kernel;
   o.check = 1;
-  check_status(brille_ir_interpolate_at(h, Q, n, &o, vals, vecs, aux, nullptr));
+  if (first_zone) check_status(brille_interpolate_at(h, Q, n, &o, vals, vecs, aux, nullptr));
+  else check_status(brille_ir_interpolate_at(h, Q, n, &o, vals, vecs, aux, nullptr));
 }
 
 }  // namespace brille_amd

@@ -73,8 +73,9 @@ class BZTrellisQdc {
   brille_grid* handle();  // create + fill on first use
   uint64_t bytes_per_point(int precision = BRILLE_FP64) const;
   // Q: n x 3 conventional rlu; outputs sized n * B * span (real) and n * B * span * 2 (complex)
+  // first_zone: BrillouinZoneTrellis3::interpolate_at (moveinto only, no rotation)
   void ir_interpolate_at(const double* Q, uint64_t n, bool no_move, int precision, int kernel, void* vals, void* vecs,
-                         const brille_aux_out* aux = nullptr);
+                         const brille_aux_out* aux = nullptr, bool first_zone = false);
 };
 
 // zone-only device handle for BrillouinZone.moveinto / ir_moveinto

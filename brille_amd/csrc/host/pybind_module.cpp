@@ -97,8 +97,8 @@ Payload payload_from(const py::array& arr, const iarr& el, bool complex_data) {
   return p;
 }
 
-py::tuple grid_ir_interpolate(PyGrid& pg, const darr& Q, bool useparallel, int threads, bool no_move,
-                              const std::string& precision, const std::string& kernel) {
+py::tuple grid_interpolate(PyGrid& pg, const darr& Q, bool useparallel, int threads, bool no_move,
+                           const std::string& precision, const std::string& kernel, bool first_zone) {
   (void)useparallel;
   (void)threads;
   if (Q.ndim() != 2 || Q.shape(1) != 3) throw std::runtime_error("Interpolation requires one or more 3-vectors");
@@ -125,7 +125,8 @@ py::tuple grid_ir_interpolate(PyGrid& pg, const darr& Q, bool useparallel, int t
   }
   {
     py::gil_scoped_release release;
-    g.ir_interpolate_at(Q.data(), n, no_move, prec, kern, vals.mutable_data(), vecs.mutable_data());
+    g.ir_interpolate_at(Q.data(), n, no_move, prec, kern, vals.mutable_data(), vecs.mutable_data(), nullptr,
+                        first_zone);
   }
   return py::make_tuple(vals, vecs);
 }
@@ -474,8 +475,21 @@ PYBIND11_MODULE(_brille, m) {
         g.vectors_ref = vecs;
       }, "values_data"_a, "values_elements"_a, "values_weights"_a, "vectors_data"_a, "vectors_elements"_a,
          "vectors_weights"_a, "sort"_a = false)
-      .def("ir_interpolate_at", &grid_ir_interpolate, "Q"_a, "useparallel"_a = false, "threads"_a = -1,
-           "do_not_move_points"_a = false, py::kw_only(), "precision"_a = "fp64", "kernel"_a = "auto")
+      .def(
+          "ir_interpolate_at",
+          [](PyGrid& pg, const darr& Q, bool par, int th, bool nm, const std::string& prec, const std::string& k) {
+            return grid_interpolate(pg, Q, par, th, nm, prec, k, false);
+          },
+          "Q"_a, "useparallel"_a = false, "threads"_a = -1, "do_not_move_points"_a = false, py::kw_only(),
+          "precision"_a = "fp64", "kernel"_a = "auto")
+      // wrap/_common_grid.hpp:412-440: first-zone fold only, no rotation
+      .def(
+          "interpolate_at",
+          [](PyGrid& pg, const darr& Q, bool par, int th, bool nm, const std::string& prec, const std::string& k) {
+            return grid_interpolate(pg, Q, par, th, nm, prec, k, true);
+          },
+          "Q"_a, "useparallel"_a = false, "threads"_a = -1, "do_not_move_points"_a = false, py::kw_only(),
+          "precision"_a = "fp64", "kernel"_a = "auto")
       .def("all_node_types", [](const PyGrid& g) { return g.g->trellis.node_type; })
       .def("node_at_type", [](const PyGrid& g, const std::array<size_t, 3>& s) {
         const auto& T = g.g->trellis;

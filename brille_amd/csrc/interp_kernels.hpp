@@ -491,7 +491,7 @@ __device__ inline void unit_generic(const GridConst& g, const DataConst& d, cons
 // One pass: every row is read once, with 16-byte loads, and contributes its
 // antiphase and its weighted elements before the next row is read.
 
-template <typename Real, int SC, int SV, bool GAMMA>
+template <typename Real, int SC, int SV, bool GAMMA, int GRP = 0>
 __device__ __forceinline__ void unit_onepass(const GridConst& g, const DataConst& d, const PointRec<Real>& r,
                                              uint32_t b, Real* ov, typename Cplx<Real>::T* oc,
                                              const typename Cplx<Real>::T* phases, bool cart) {
@@ -511,7 +511,7 @@ __device__ __forceinline__ void unit_onepass(const GridConst& g, const DataConst
   // wave keeps GR rows per lane in flight (8 in FP32, 4 in FP64 for the
   // register budget).  Unused tetrahedron slots repeat the pivot with weight 0:
   // their loads hit in cache and add exact zeros, and no branch splits the loads.
-  constexpr int GR = sizeof(Real) == 4 ? 8 : 4;
+  constexpr int GR = GRP ? GRP : (sizeof(Real) == 4 ? 8 : 4);
 #pragma unroll
   for (int x0 = 0; x0 < 8; x0 += GR) {
     Real row[GR][NR];
@@ -669,7 +669,7 @@ struct LaunchArgs {
   void* vecs;
   brille_aux_out aux;
   ErrRec* err;
-  int no_move;
+  int fold;  // Fold
   int use_proposal;
   ProposalConst prop;
 };
@@ -677,9 +677,15 @@ struct LaunchArgs {
 // ---------------------------------------------------------------------------
 // phase 1 for one point
 
+// How stage (a) folds a point: into the irreducible wedge (ir_interpolate_at),
+// not at all (do_not_move_points), or into the first zone only, with the
+// identity operation (interpolate_at, src/bz_trellis.hpp:111-128: the identity
+// rotation and a zero Gamma phase leave the sums exactly as they are).
+enum Fold : int { FOLD_IR = 0, FOLD_NONE = 1, FOLD_FIRST_ZONE = 2 };
+
 // stage (a) and the node subscript of stage (b): q, tau, Ridx/invRidx, node
 template <bool Fast>
-__device__ inline void fold_node(const GridConst& g, const double* Q, bool no_move, Located& L) {
+__device__ inline void fold_node(const GridConst& g, const double* Q, int fold, Located& L) {
   const Tol tbz{g.tol_bz_rel, g.tol_bz_abs};
   const Tol tdef{g.tol_def_rel, g.tol_def_abs};
   L.tau[0] = L.tau[1] = L.tau[2] = 0;
@@ -690,12 +696,13 @@ __device__ inline void fold_node(const GridConst& g, const double* Q, bool no_mo
   L.nkept = 0;
   L.pivot = 0;
   L.status = BRILLE_PT_OK;
-  if (no_move) {
+  if (fold == FOLD_NONE) {
     L.q[0] = Q[0]; L.q[1] = Q[1]; L.q[2] = Q[2];
   } else {
     int st = Fast ? fast_moveinto(g, Q, L.q, L.tau, tbz) : moveinto_exact(g, Q, L.q, L.tau, tbz);
-    if (st == BRILLE_PT_OK) st = Fast ? fast_wedge_search(g, L.q, &L.rot, &L.invrot, tbz, tdef)
-                                      : wedge_search_exact(g, L.q, &L.rot, &L.invrot, tbz, tdef);
+    if (st == BRILLE_PT_OK && fold == FOLD_IR)
+      st = Fast ? fast_wedge_search(g, L.q, &L.rot, &L.invrot, tbz, tdef)
+                : wedge_search_exact(g, L.q, &L.rot, &L.invrot, tbz, tdef);
     if (st != BRILLE_PT_OK) { L.status = st; return; }
   }
   double x[3];
@@ -758,8 +765,8 @@ __device__ inline void node_slots(const GridConst& g, Located& L, uint32_t hint 
 }
 
 template <bool Fast>
-__device__ inline void locate(const GridConst& g, const double* Q, bool no_move, Located& L) {
-  fold_node<Fast>(g, Q, no_move, L);
+__device__ inline void locate(const GridConst& g, const double* Q, int fold, Located& L) {
+  fold_node<Fast>(g, Q, fold, L);
   if (L.status == BRILLE_PT_OK) node_slots(g, L);
 }
 
@@ -829,8 +836,9 @@ __device__ __forceinline__ void store_record(const DataConst& d, const Located& 
 }
 
 // phase 2 over the npts records of a tile of TB points (TB threads): point pl
-// writes output row out_row[pl] (or row0 + pl when out_row is null)
-template <typename Real, bool Fast, int L, int SC, int SV, int TB = TILE>
+// writes output row out_row[pl] (or row0 + pl when out_row is null); GRP
+// overrides the rows the one-pass sums keep in flight (0: their default)
+template <typename Real, bool Fast, int L, int SC, int SV, int TB = TILE, int GRP = 0>
 __device__ __forceinline__ void tile_sums(const GridConst& g, const DataConst& d, const PointRec<Real>* s_rec,
                                           const typename Cplx<Real>::T* s_ph, const double* s_q,
                                           const uint32_t* out_row, uint64_t row0, uint32_t npts, bool ph_in_lds,
@@ -875,9 +883,9 @@ __device__ __forceinline__ void tile_sums(const GridConst& g, const DataConst& d
       continue;
     }
     if constexpr (L == LAYOUT_GAMMA) {
-      unit_onepass<Real, SC, SV, true>(g, d, r, b, ov, oc, s_ph + NPH * pl, cart);
+      unit_onepass<Real, SC, SV, true, GRP>(g, d, r, b, ov, oc, s_ph + NPH * pl, cart);
     } else if constexpr (L == LAYOUT_SCALAR) {
-      unit_onepass<Real, SC, SV, false>(g, d, r, b, ov, oc, nullptr, cart);
+      unit_onepass<Real, SC, SV, false, GRP>(g, d, r, b, ov, oc, nullptr, cart);
     } else {
       unit_generic<Real, !Fast>(g, d, r, s_q + 3 * pl, b, ov, oc, ph_in_lds ? s_ph + NPH * pl : nullptr);
     }
@@ -916,7 +924,7 @@ __global__ __launch_bounds__(TILE) void k_ir_interpolate(GridConst g, DataConst 
     P.status = BRILLE_PT_OK;
     if (live) {
       double Qp[3] = {a.Q[3 * p], a.Q[3 * p + 1], a.Q[3 * p + 2]};
-      fold_node<Fast>(g, Qp, a.no_move, P);
+      fold_node<Fast>(g, Qp, a.fold, P);
     }
     if (live && P.status == BRILLE_PT_OK) node_slots(g, P);
     if (live) {

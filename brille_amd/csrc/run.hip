@@ -182,7 +182,7 @@ static int read_proposal_stats(brille_grid* G, hipStream_t s) {
 
 int run_interpolate(brille_grid* G, const double* Q, uint64_t n, const brille_run_opts* opts_in, void* values_out,
                     void* vectors_out, const brille_aux_out* aux_in, hipStream_t s, brille_kernel_times* times,
-                    uint32_t iters) {
+                    uint32_t iters, int fold) {
   if (!G) return fail(BRILLE_ERR_INVALID_ARGUMENT, "null grid");
   if (!G->filled) return fail(BRILLE_ERR_NOT_FILLED, "The interpolation data must be filled before interpolating.");
   if (G->g.n_nodes == 0) return fail(BRILLE_ERR_INVALID_ARGUMENT, "this handle holds no trellis");
@@ -208,7 +208,7 @@ int run_interpolate(brille_grid* G, const double* Q, uint64_t n, const brille_ru
   brille_aux_out aux{};
   if (aux_in) aux = *aux_in;
   LaunchArgs a{};
-  a.no_move = opts.no_move;
+  a.fold = opts.no_move ? FOLD_NONE : fold;
   a.use_proposal = opts.use_proposal;
   a.prop = G->prop;
   if (opts.use_proposal) HIP_TRY(hipMemsetAsync(G->prop_stats, 0, 2 * sizeof(unsigned long long), s));

@@ -242,6 +242,14 @@ int brille_ir_interpolate_at(brille_grid* grid, const double* Q, uint64_t n,
                              const brille_run_opts* opts, void* values_out,
                              void* vectors_out, const brille_aux_out* aux,
                              void* stream);
+/* BrillouinZoneTrellis3::interpolate_at  src/bz_trellis.hpp:111-128 (Python
+ * BZTrellisQdc.interpolate_at, wrap/_common_grid.hpp:412-440): Q is moved into
+ * the first Brillouin zone only (BrillouinZone::moveinto, no wedge search) and
+ * interpolated with no rotation; the trellis covers the irreducible wedge, so
+ * first-zone images outside it fail as in the reference.  Same arguments. */
+int brille_interpolate_at(brille_grid* grid, const double* Q, uint64_t n,
+                          const brille_run_opts* opts, void* values_out,
+                          void* vectors_out, const brille_aux_out* aux, void* stream);
 /* stage (a) alone: q, tau, Ridx, invRidx  src/bz_move.cpp:165-296 */
 int brille_ir_moveinto(brille_grid* grid, const double* Q, uint64_t n, int32_t memory,
                        double* q_out, int32_t* tau_out, uint32_t* rot_out,

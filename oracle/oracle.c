@@ -615,6 +615,9 @@ static int rotate_point(const brille_grid_desc* g, const brille_data_desc* d,
  * BrillouinZoneTrellis3::ir_interpolate_at  src/bz_trellis.hpp:153-196
  * values_out: N x B x span (double); vectors_out: N x B x span complex (interleaved).
  * Returns a brille_status; per-point status in aux->status when given.
+ * no_move: 0 fold into the irreducible wedge; 1 do_not_move_points; 2 the
+ * first-zone fold of BrillouinZoneTrellis3::interpolate_at (src/bz_trellis.hpp:
+ * 111-128: moveinto, then the trellis interpolation with no rotation).
  */
 int brille_oracle_ir_interpolate_at(const brille_grid_desc* g, const brille_data_desc* vals,
                                     const brille_data_desc* vecs, const brille_gamma_desc* gt,
@@ -640,8 +643,13 @@ int brille_oracle_ir_interpolate_at(const brille_grid_desc* g, const brille_data
 
   /* stage (a): serial, src/bz_move.cpp:135-141, 262-285 */
   for (uint64_t i = 0; i < n; ++i) {
-    if (no_move) {
+    if (no_move == 1) {
       for (int k = 0; k < 3; ++k) { irq[3 * i + k] = Q[3 * i + k]; tau[3 * i + k] = 0; }
+      rot[i] = invrot[i] = g->identity_index;
+      continue;
+    }
+    if (no_move == 2) {
+      status[i] = (uint8_t)moveinto(g, Q + 3 * i, irq + 3 * i, tau + 3 * i, tbz);
       rot[i] = invrot[i] = g->identity_index;
       continue;
     }
@@ -710,10 +718,10 @@ int brille_oracle_ir_interpolate_at(const brille_grid_desc* g, const brille_data
     }
   }
 
-  /* stage (c2): src/bz_trellis.hpp:174-192 */
+  /* stage (c2): src/bz_trellis.hpp:174-192 (interpolate_at has none) */
   int rot_status = BRILLE_OK;
-#pragma omp parallel num_threads(nthreads)
-  {
+#pragma omp parallel num_threads(nthreads) if (no_move != 2)
+  if (no_move != 2) {
     uint32_t maxs = sc > sv ? sc : sv;
     double* tA = (double*)malloc(sizeof(double) * 2 * (maxs + 3));
     int local = BRILLE_OK;

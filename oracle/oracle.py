@@ -53,8 +53,9 @@ def _d(a):
     return np.ascontiguousarray(a, dtype=np.float64)
 
 
-def ir_interpolate_at(tables, Q, no_move=False, nthreads=1, values=None, vectors=None, perm=None):
-    """Returns (rc, values, vectors, aux, first_bad) computed on the CPU."""
+def ir_interpolate_at(tables, Q, no_move=False, nthreads=1, values=None, vectors=None, perm=None, first_zone=False):
+    """Returns (rc, values, vectors, aux, first_bad) computed on the CPU.
+    first_zone: BrillouinZoneTrellis3::interpolate_at instead (moveinto only, no rotation)."""
     th = TableHolder(tables)
     dh = DataHolder(tables, values, vectors, perm)
     Q = _d(Q).reshape(-1, 3)
@@ -69,7 +70,8 @@ def ir_interpolate_at(tables, Q, no_move=False, nthreads=1, values=None, vectors
     bad = C.c_uint64(0)
     rc = lib().brille_oracle_ir_interpolate_at(
         C.byref(th.desc), C.byref(dh.vdesc), C.byref(dh.cdesc), C.byref(dh.gdesc) if dh.gdesc is not None else None,
-        C.byref(dh.pdesc) if dh.pdesc is not None else None, Q.ctypes.data_as(_dp), n, int(no_move), int(nthreads),
+        C.byref(dh.pdesc) if dh.pdesc is not None else None, Q.ctypes.data_as(_dp), n,
+        1 if no_move else (2 if first_zone else 0), int(nthreads),
         vals.ctypes.data_as(_dp), vecs.ctypes.data_as(_dp), C.byref(aux), C.byref(bad))
     return rc, vals, vecs, bufs, bad.value
 

@@ -191,3 +191,33 @@ def test_ir_moveinto_entry_point():
     assert rc == orc == 0
     for a, b_ in ((q, oq), (tau, otau), (r, orr), (ir, oir), (st, ost)):
         assert np.array_equal(a, b_)
+
+
+@pytest.mark.parametrize("kernel", [capi.KERNEL_PLAIN, capi.KERNEL_TILE, capi.KERNEL_BINNED])
+def test_first_zone_interpolate_at(kernel):
+    """brille_interpolate_at (BrillouinZoneTrellis3::interpolate_at: moveinto only,
+    no rotation) against the oracle's first-zone mode."""
+    say(f"interpolate_at kernel={kernel}")
+    g = S.grid(1)
+    t = g.tables()
+    dev = capi.Grid(t)
+    bz = g.BrillouinZone
+    ir = np.asarray(bz.ir_vertices)
+    r = np.random.default_rng(8)
+    q = r.dirichlet(np.ones(len(ir)), size=5000) @ ir
+    Q = q + r.integers(-2, 3, size=(5000, 3))
+    aux = dict(tau=False) if kernel == capi.KERNEL_BINNED else True
+    gv, gc, ga = dev.interpolate_at(Q, precision=capi.FP64, kernel=kernel, aux=aux)
+    rc, ov, oc, oa, _ = O.ir_interpolate_at(t, Q, nthreads=8, first_zone=True)
+    assert rc == 0
+    for k in ga:
+        np.testing.assert_array_equal(ga[k], oa[k], err_msg=k)
+    np.testing.assert_allclose(gv, ov, rtol=1e-12, atol=1e-12 * np.abs(ov).max())
+    np.testing.assert_allclose(gc, oc, rtol=1e-12, atol=1e-12)
+    # first-zone images outside the wedge fail as in the reference, at the same first point
+    Qb = S.q_points(1, 3000, stream=42)
+    rc, *_ , bad = O.ir_interpolate_at(t, Qb, nthreads=8, first_zone=True)
+    with pytest.raises(capi.BrilleError) as e:
+        dev.interpolate_at(Qb, precision=capi.FP32, kernel=kernel)
+    assert e.value.status == "NULL_NODE"
+    assert f"{Qb[bad][0]:.17g}"[:12] in str(e.value) or str(bad) in str(e.value)

@@ -227,3 +227,44 @@ def test_permutation_table_is_applied():
     assert rc == 0
     assert np.allclose(v1[0], v0[0], rtol=1e-13, atol=1e-13)
     assert np.allclose(c1[0], c0[0], rtol=1e-12, atol=1e-12)
+
+
+def _first_zone_points(g, n, seed):
+    """Points whose first-zone image is inside the irreducible wedge: random
+    interior points of the irreducible polyhedron, shifted by lattice vectors."""
+    bz = g.BrillouinZone
+    ir = np.asarray(bz.ir_vertices)
+    r = np.random.default_rng(seed)
+    w = r.dirichlet(np.ones(len(ir)), size=n)
+    q = w @ ir
+    tau = r.integers(-2, 3, size=(n, 3))
+    return q, q + tau
+
+
+def test_first_zone_mode_matches_the_irreducible_path_on_wedge_images():
+    """interpolate_at (moveinto, no rotation) equals ir_interpolate_at wherever the
+    wedge search picks the identity, and its q is BrillouinZone::moveinto's."""
+    from brille_amd import synthetic as S
+    g = S.grid(1)
+    t = g.tables()
+    q, Q = _first_zone_points(g, 3000, 5)
+    rc1, v1, c1, a1, _ = O.ir_interpolate_at(t, Q, first_zone=True)
+    rc0, v0, c0, a0, _ = O.ir_interpolate_at(t, Q)
+    assert rc0 == 0 and rc1 == 0
+    assert np.all(a0["rot"] == t["identity_index"]) and np.all(a1["rot"] == t["identity_index"])
+    np.testing.assert_array_equal(v1, v0)
+    np.testing.assert_array_equal(c1, c0)
+    rc, qm, taum, st = O.moveinto(t, Q)
+    np.testing.assert_array_equal(a1["ir_q"], qm)
+    np.testing.assert_array_equal(a1["tau"], taum)
+    np.testing.assert_allclose(a1["ir_q"], q, atol=1e-12)
+
+
+def test_first_zone_mode_fails_outside_the_wedge():
+    from brille_amd import synthetic as S
+    g = S.grid(1)
+    t = g.tables()
+    Q = S.q_points(1, 2000, stream=41)
+    rc, v, c, aux, bad = O.ir_interpolate_at(t, Q, first_zone=True)
+    assert rc == 5  # BRILLE_ERR_NULL_NODE: first-zone images outside the trellis
+    assert aux["status"][bad] == 3  # BRILLE_PT_NULL_NODE
