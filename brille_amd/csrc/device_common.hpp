@@ -109,8 +109,6 @@ constexpr int kPropH = 128;
 struct ProposalConst {
   int32_t enabled;
   uint32_t K;
-  float lo[3];     // features: (xyz - lo) * scale - 1, in [-1, 1] over the trellis box
-  float scale[3];
   const float* W1;     // H x 3
   const float* b1;     // H
   const uint16_t* W2;  // H x H bf16, row = output unit

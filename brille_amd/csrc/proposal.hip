@@ -70,7 +70,7 @@ __global__ __launch_bounds__(256) void k_prop_label(GridConst g, ProposalConst p
 #pragma unroll
   for (int d = 0; d < 3; ++d) Q[d] = 4.0 * unit_double(splitmix64(seed * 0x632BE59BD9B4E019ull + 3 * id + d)) - 2.0;
   Located L;
-  fold_node<true>(g, Q, false, L);
+  fold_node<true>(g, Q, FOLD_IR, L);
   if (L.status != BRILLE_PT_OK || g.node_type[L.node] != BRILLE_NODE_POLY) return;
   node_slots(g, L);
   if (L.status != BRILLE_PT_OK || L.tet == kNoTet) return;
@@ -82,7 +82,7 @@ __global__ __launch_bounds__(256) void k_prop_label(GridConst g, ProposalConst p
   double x[3];
   mat_vec(x, g.outer_to_xyz, L.q);
   float f[3];
-  prop_features(pc, x, f);
+  prop_features(g, x, L.node, f);
   feat[3 * (uint64_t)slot] = f[0];
   feat[3 * (uint64_t)slot + 1] = f[1];
   feat[3 * (uint64_t)slot + 2] = f[2];
@@ -331,12 +331,6 @@ static int prop_setup(brille_grid* G, uint64_t seed) {
   P_TRY(hipGetLastError());
   ProposalConst& pc = G->prop;
   pc.K = K;
-  for (int d = 0; d < 3; ++d) {
-    const double lo = G->g.knot_lo[d];
-    const double hi = lo + (double)(G->g.nk[d] - 1) / G->g.knot_inv_step[d];
-    pc.lo[d] = (float)lo;
-    pc.scale[d] = (float)(2.0 / (hi - lo));
-  }
   pc.W1 = G->prop_params + o.W1;
   pc.b1 = G->prop_params + o.b1;
   pc.W2 = W2b;
@@ -404,7 +398,7 @@ int brille_proposal_train(brille_grid* G, const brille_train_opts* opts_in, bril
   brille_train_opts o{};
   if (opts_in) o = *opts_in;
   if (o.n_train == 0) o.n_train = 20000000ull;
-  if (o.epochs == 0) o.epochs = 4;
+  if (o.epochs == 0) o.epochs = 8;
   if (o.batch == 0) o.batch = 8192;
   if (!(o.learning_rate > 0.f)) o.learning_rate = 2e-3f;
   if (o.n_train > (1ull << 31)) return fail(BRILLE_ERR_INVALID_ARGUMENT, "n_train too large");

@@ -26,10 +26,17 @@ namespace brille_amd {
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float floatx4_t __attribute__((ext_vector_type(4)));
 
-// network input of a point: its Cartesian coordinates scaled into [-1, 1]
-__device__ __forceinline__ void prop_features(const ProposalConst& pc, const double* x, float* f) {
+// network input of a point: its Cartesian coordinates inside its trellis
+// node, scaled into [-1, 1] (a node's tetrahedra are a fan from one apex, so
+// their boundaries are a few planes in these coordinates)
+__device__ __forceinline__ void prop_features(const GridConst& g, const double* x, uint32_t node, float* f) {
+  const uint32_t s0 = g.nk[0] - 1, s1 = g.nk[1] - 1;
+  const uint32_t sub[3] = {node % s0, (node / s0) % s1, node / (s0 * s1)};
 #pragma unroll
-  for (int d = 0; d < 3; ++d) f[d] = (float)((x[d] - (double)pc.lo[d]) * (double)pc.scale[d] - 1.0);
+  for (int d = 0; d < 3; ++d) {
+    const double lo = g.knots[d][sub[d]], hi = g.knots[d][sub[d] + 1];
+    f[d] = (float)(2.0 * (x[d] - lo) / (hi - lo) - 1.0);
+  }
 }
 
 // Proposals for the n <= 16 points of a chunk (feat: 3 floats per point,
@@ -148,7 +155,7 @@ __device__ inline uint32_t propose_block(const GridConst& g, const ProposalConst
     double x[3];
     mat_vec(x, g.outer_to_xyz, q);
     float f[3];
-    prop_features(pc, x, f);
+    prop_features(g, x, node, f);
     pq.feat[3 * slot] = f[0];
     pq.feat[3 * slot + 1] = f[1];
     pq.feat[3 * slot + 2] = f[2];

@@ -304,9 +304,10 @@ int brille_proposal_loss_grad(brille_grid* grid, const float* x, const int32_t* 
                               uint32_t n, float* loss, float* grads, uint32_t* n_params);
 int brille_proposal_get_params(const brille_grid* grid, float* params, uint32_t* n_params);
 int brille_proposal_set_params(brille_grid* grid, const float* params, uint32_t n_params);
-/* the network's proposal for points given by their features (n x 3 float, the
- * scaled Cartesian coordinates) and any tetrahedron of their polyhedron node
- * (in_tet selects the candidates); host memory */
+/* the network's proposal for points given by their features (n x 3 float: the
+ * Cartesian coordinates inside the point's trellis node, scaled to [-1, 1])
+ * and any tetrahedron of their polyhedron node (in_tet selects the
+ * candidates); host memory */
 int brille_proposal_predict(brille_grid* grid, const float* x, const int32_t* in_tet, uint32_t n,
                             int32_t* tet_out);
 

@@ -111,7 +111,7 @@ def test_trained_proposal_keeps_the_reference_choice():
     dev = capi.Grid(t)
     lib = capi.lib()
     say(f"training the proposal: {t['tet_volume'].size} tetrahedra")
-    opts = capi.TrainOpts(2_000_000, 7, 3, 8192, 2e-3)
+    opts = capi.TrainOpts(2_000_000, 7, 8, 8192, 2e-3)
     rep = capi.TrainReport()
     capi.check(lib.brille_proposal_train(dev.h, C.byref(opts), C.byref(rep)))
     say(f"trained on {rep.n_pairs} pairs in {rep.ms:.0f} ms: loss {rep.final_loss:.4f}, "
