@@ -32,8 +32,10 @@ constexpr uint32_t SCAN_SEG = 4096;                   // elements per scan block
 struct FoldRec {
   double q[3];
   uint32_t node;
-  uint32_t meta;  // rot | invrot << 8 | status << 16
+  uint32_t meta;  // rot | invrot << 8 | status << 16 | kMetaPoly
 };
+constexpr uint32_t kMetaPoly = 1u << 24;  // the point folded into a polyhedron node
+__device__ __forceinline__ uint32_t meta_status(uint32_t meta) { return (meta >> 16) & 255u; }
 struct SortRec {
   double q[3];
   uint32_t node;
@@ -43,30 +45,47 @@ struct SortRec {
 };
 
 __device__ __forceinline__ uint32_t bucket_of(uint32_t node, uint32_t meta, uint32_t nb, uint64_t n_nodes) {
-  return (meta >> 16) != BRILLE_PT_OK ? nb : (uint32_t)(((uint64_t)node * nb) / n_nodes);
+  return meta_status(meta) != BRILLE_PT_OK ? nb : (uint32_t)(((uint64_t)node * nb) / n_nodes);
 }
 
+// poly_list/poly_count (nullable): the points that folded into polyhedron
+// nodes, for the learned proposal (appended per wave: ballot, prefix count,
+// one atomic)
 template <bool Fast>
 __global__ __launch_bounds__(BIN_BLOCK) void k_bin_fold(GridConst g, const double* Q, uint32_t n, int fold,
                                                         FoldRec* rec, uint32_t* hist, uint32_t nblk, uint32_t nb,
-                                                        ErrRec* err, uint64_t base) {
+                                                        ErrRec* err, uint64_t base, uint32_t* poly_list,
+                                                        uint32_t* poly_count) {
   extern __shared__ uint32_t h[];  // nb + 1 counters
   for (uint32_t k = threadIdx.x; k <= nb; k += BIN_BLOCK) h[k] = 0;
   __syncthreads();
   const uint32_t b0 = blockIdx.x * BIN_SPAN;
+  const uint32_t lane = threadIdx.x & 63u;
   for (int i = 0; i < BIN_PPT; ++i) {
     const uint32_t p = b0 + i * BIN_BLOCK + threadIdx.x;
+    bool poly = false;
     if (p < n) {
       double Qp[3] = {Q[3 * (uint64_t)p], Q[3 * (uint64_t)p + 1], Q[3 * (uint64_t)p + 2]};
       Located L;
       fold_node<Fast>(g, Qp, fold, L);
       if (L.status != BRILLE_PT_OK) report_point(err, base + p, L.status);
+      poly = L.status == BRILLE_PT_OK && g.node_type[L.node] == BRILLE_NODE_POLY;
       FoldRec r;
       r.q[0] = L.q[0]; r.q[1] = L.q[1]; r.q[2] = L.q[2];
       r.node = L.node;
-      r.meta = (L.rot & 255u) | ((L.invrot & 255u) << 8) | ((uint32_t)L.status << 16);
+      r.meta = (L.rot & 255u) | ((L.invrot & 255u) << 8) | ((uint32_t)L.status << 16) | (poly ? kMetaPoly : 0u);
       rec[p] = r;
       atomicAdd(&h[bucket_of(r.node, r.meta, nb, g.n_nodes)], 1u);
+    }
+    if (poly_list) {
+      const unsigned long long mask = __ballot(poly);
+      if (mask) {
+        const uint32_t leader = (uint32_t)__ffsll((long long)mask) - 1u;
+        uint32_t at = 0;
+        if (lane == leader) at = atomicAdd(poly_count, (uint32_t)__popcll(mask));
+        at = __shfl(at, (int)leader);
+        if (poly) poly_list[at + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull))] = p;
+      }
     }
   }
   __syncthreads();
@@ -146,7 +165,8 @@ __global__ __launch_bounds__(BIN_BLOCK) void k_scan_apply(uint32_t* a, uint64_t 
 }
 
 __global__ __launch_bounds__(BIN_BLOCK) void k_bin_scatter(const FoldRec* rec, uint32_t n, const uint32_t* off,
-                                                           uint32_t nblk, uint32_t nb, uint64_t n_nodes, SortRec* out) {
+                                                           uint32_t nblk, uint32_t nb, uint64_t n_nodes, SortRec* out,
+                                                           const uint32_t* hint) {
   extern __shared__ uint32_t s[];  // base[nb + 1], count[nb + 1]
   uint32_t* base = s;
   uint32_t* cnt = s + nb + 1;
@@ -167,29 +187,52 @@ __global__ __launch_bounds__(BIN_BLOCK) void k_bin_scatter(const FoldRec* rec, u
       o.node = r.node;
       o.meta = r.meta;
       o.index = p;
-      o.hint = kNoTet;
+      o.hint = (hint && (r.meta & kMetaPoly)) ? hint[p] : kNoTet;
       out[base[k] + j] = o;
     }
   }
 }
 
-// the learned proposal over bucket-ordered records: the points in polyhedron
-// nodes are compacted per tile and the network's choice is stored in the
-// record for k_bin_sums to verify
-__global__ __launch_bounds__(TILE) void k_bin_propose(GridConst g, ProposalConst pc, SortRec* srt, uint32_t n) {
-  __shared__ PropQueue<TILE> s_pq;
-  const uint32_t p = blockIdx.x * TILE + threadIdx.x;
-  bool want = false;
-  double q[3] = {0.0, 0.0, 0.0};
-  uint32_t node = 0;
-  if (p < n) {
-    const SortRec& sr = srt[p];
-    node = sr.node;
-    want = (sr.meta >> 16) == BRILLE_PT_OK && g.node_type[node] == BRILLE_NODE_POLY;
-    q[0] = sr.q[0]; q[1] = sr.q[1]; q[2] = sr.q[2];
+// The learned proposal for the listed points only (those in polyhedron
+// nodes): each wave takes 16 list entries at a time, stages their features
+// and candidate ranges in its LDS slice, runs the network and writes the
+// proposed tetrahedron to hint[point].  Grid-stride over the list, whose
+// length is read on the device.
+__global__ __launch_bounds__(256) void k_bin_propose(GridConst g, ProposalConst pc, const FoldRec* rec,
+                                                     const uint32_t* list, const uint32_t* count, uint32_t* hint) {
+  __shared__ float s_feat[4][48];
+  __shared__ uint32_t s_t0[4][16], s_nc[4][16], s_out[4][16];
+  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+  const uint32_t n = *count;
+  const uint32_t waves = gridDim.x * 4u;
+  for (uint32_t c0 = (blockIdx.x * 4u + wave) * 16u; c0 < n; c0 += waves * 16u) {
+    const uint32_t k = n - c0 < 16u ? n - c0 : 16u;
+    if (lane < k) {
+      const FoldRec r = rec[list[c0 + lane]];
+      double x[3];
+      mat_vec(x, g.outer_to_xyz, r.q);
+      float f[3];
+      prop_features(g, x, r.node, f);
+      s_feat[wave][3 * lane] = f[0];
+      s_feat[wave][3 * lane + 1] = f[1];
+      s_feat[wave][3 * lane + 2] = f[2];
+      const uint32_t pi = g.node_index[r.node];
+      const uint32_t a = g.poly_off[pi];
+      s_t0[wave][lane] = a;
+      s_nc[wave][lane] = g.poly_off[pi + 1] - a;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    propose16(pc, s_feat[wave], s_t0[wave], s_nc[wave], k, s_out[wave]);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (lane < k) hint[list[c0 + lane]] = s_out[wave][lane];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
-  const uint32_t hint = propose_block<TILE>(g, pc, s_pq, want, q, node);
-  if (want) srt[p].hint = hint;
 }
 
 template <typename Real, int L, int SC, int SV, bool Prop>
@@ -227,7 +270,7 @@ __global__ __launch_bounds__(TILE) void k_bin_sums(GridConst g, DataConst d, Pro
       P.node = sr.node;
       P.rot = sr.meta & 255u;
       P.invrot = (sr.meta >> 8) & 255u;
-      P.status = (int)(sr.meta >> 16);
+      P.status = (int)meta_status(sr.meta);
       P.tet = kNoTet;
       P.nslot = 0;
       P.nkept = 0;
@@ -352,7 +395,8 @@ int launch_binned(brille_grid* G, const LaunchArgs& a, int precision, hipStream_
   const uint64_t np_max = (m_max + SCAN_SEG - 1) / SCAN_SEG;
   auto al = [](uint64_t b) { return (b + 255) & ~uint64_t(255); };
   const uint64_t o_rec = 0, o_srt = al(chunk * sizeof(FoldRec)), o_hist = o_srt + al(chunk * sizeof(SortRec)),
-                 o_part = o_hist + al(m_max * 4), need = o_part + al(np_max * 4);
+                 o_part = o_hist + al(m_max * 4), o_list = o_part + al(np_max * 4), o_hint = o_list + al(chunk * 4),
+                 o_cnt = o_hint + al(chunk * 4), need = o_cnt + 256;
   if (np_max > 256ull * 1024) return fail(BRILLE_ERR_INVALID_ARGUMENT, "binned pass too large");
   if (G->bin_ws_bytes < need) {
     free_binned(G);
@@ -365,6 +409,9 @@ int launch_binned(brille_grid* G, const LaunchArgs& a, int precision, hipStream_
   uint32_t* hist = reinterpret_cast<uint32_t*>(ws + o_hist);
   uint32_t* part = reinterpret_cast<uint32_t*>(ws + o_part);
   const bool prop = a.use_proposal != 0;
+  uint32_t* plist = prop ? reinterpret_cast<uint32_t*>(ws + o_list) : nullptr;
+  uint32_t* phint = prop ? reinterpret_cast<uint32_t*>(ws + o_hint) : nullptr;
+  uint32_t* pcount = reinterpret_cast<uint32_t*>(ws + o_cnt);
   const SumsKernel sums = precision == BRILLE_FP32 ? pick_sums<float>(G->d, prop) : pick_sums<double>(G->d, prop);
   const size_t rs = precision == BRILLE_FP32 ? 4 : 8;
   const size_t vb = (size_t)G->d.B * G->d.sv * rs, cb = (size_t)G->d.B * G->d.sc * 2 * rs;
@@ -374,18 +421,21 @@ int launch_binned(brille_grid* G, const LaunchArgs& a, int precision, hipStream_
     const uint64_t M = (uint64_t)(nb + 1) * nblk;
     const uint32_t np = (uint32_t)((M + SCAN_SEG - 1) / SCAN_SEG);
     const size_t lds1 = (size_t)(nb + 1) * 4;
+    if (prop && hipMemsetAsync(pcount, 0, sizeof(uint32_t), s) != hipSuccess)
+      return fail(BRILLE_ERR_DEVICE, "hipMemsetAsync failed");
     if (evp) (void)hipEventRecord(evp[0], s);
     hipLaunchKernelGGL(k_bin_fold<true>, dim3(nblk), dim3(BIN_BLOCK), lds1, s, G->g, a.Q + 3 * c0, m, a.fold, rec,
-                       hist, nblk, nb, a.err, a.base + c0);
+                       hist, nblk, nb, a.err, a.base + c0, plist, pcount);
     if (evp) (void)hipEventRecord(evp[1], s);
     hipLaunchKernelGGL(k_scan_reduce, dim3(np), dim3(BIN_BLOCK), 0, s, hist, M, part);
     hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(BIN_BLOCK), 0, s, part, np);
     hipLaunchKernelGGL(k_scan_apply, dim3(np), dim3(BIN_BLOCK), 0, s, hist, M, part);
-    hipLaunchKernelGGL(k_bin_scatter, dim3(nblk), dim3(BIN_BLOCK), 2 * lds1, s, rec, m, hist, nblk, nb,
-                       G->g.n_nodes, srt);
-    if (evp) (void)hipEventRecord(evp[2], s);
     if (prop)
-      hipLaunchKernelGGL(k_bin_propose, dim3((m + TILE - 1) / TILE), dim3(TILE), 0, s, G->g, a.prop, srt, m);
+      hipLaunchKernelGGL(k_bin_propose, dim3(G->n_cu ? 4 * G->n_cu : 1024), dim3(256), 0, s, G->g, a.prop, rec, plist,
+                         pcount, phint);
+    hipLaunchKernelGGL(k_bin_scatter, dim3(nblk), dim3(BIN_BLOCK), 2 * lds1, s, rec, m, hist, nblk, nb,
+                       G->g.n_nodes, srt, phint);
+    if (evp) (void)hipEventRecord(evp[2], s);
     hipLaunchKernelGGL(sums, dim3((m + TILE - 1) / TILE), dim3(TILE), 0, s, G->g, G->d, a.prop, srt, m,
                        static_cast<char*>(a.vals) + c0 * vb, static_cast<char*>(a.vecs) + c0 * cb,
                        offset_aux(a.aux, c0), a.err, a.base + c0);

@@ -260,6 +260,12 @@ int brille_grid_create(const brille_grid_desc* desc, brille_grid** out) {
   }
   G->n_vertices = desc->n_vertices;
   G->n_polys = desc->n_polys;
+  {
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+      G->n_cu = (uint32_t)prop.multiProcessorCount;
+  }
   G->n_tets = desc->n_tets;
   void* e = nullptr;
   if (hipMalloc(&e, sizeof(ErrRec)) != hipSuccess) {

@@ -24,6 +24,7 @@ struct brille_grid {
   // work space of the binned pipeline (grown on demand, kept)
   void* bin_ws = nullptr;
   size_t bin_ws_bytes = 0;
+  uint32_t n_cu = 0;  // compute units of the device (grid sizes of grid-stride kernels)
   // learned tetrahedron proposal: FP32 master parameters (flat: W1 b1 W2 b2
   // W3 b3), Adam moments and gradients; bf16 inference copies in `prop`
   brille_amd::ProposalConst prop{};

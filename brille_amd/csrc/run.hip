@@ -230,7 +230,7 @@ int run_interpolate(brille_grid* G, const double* Q, uint64_t n, const brille_ru
           rc = launch_binned(G, a, opts.precision, s, st);
           if (rc != BRILLE_OK) return rc;
         }
-        const char* nm[3] = {"k_bin_fold", "k_bin_bucket", opts.use_proposal ? "k_bin_propose+sums" : "k_bin_sums"};
+        const char* nm[3] = {"k_bin_fold", opts.use_proposal ? "k_bin_bucket+propose" : "k_bin_bucket", "k_bin_sums"};
         times->n_kernels = 3;
         for (int k = 0; k < 3; ++k) {
           std::snprintf(times->names[k], sizeof(times->names[k]), "%s", nm[k]);
