@@ -57,7 +57,10 @@ __global__ __launch_bounds__(BIN_BLOCK) void k_bin_fold(GridConst g, const doubl
                                                         ErrRec* err, uint64_t base, uint32_t* poly_list,
                                                         uint32_t* poly_count) {
   extern __shared__ uint32_t h[];  // nb + 1 counters
+  __shared__ uint32_t s_poly[BIN_SPAN];
+  __shared__ uint32_t s_npoly, s_at;
   for (uint32_t k = threadIdx.x; k <= nb; k += BIN_BLOCK) h[k] = 0;
+  if (threadIdx.x == 0) s_npoly = 0;
   __syncthreads();
   const uint32_t b0 = blockIdx.x * BIN_SPAN;
   const uint32_t lane = threadIdx.x & 63u;
@@ -78,18 +81,24 @@ __global__ __launch_bounds__(BIN_BLOCK) void k_bin_fold(GridConst g, const doubl
       atomicAdd(&h[bucket_of(r.node, r.meta, nb, g.n_nodes)], 1u);
     }
     if (poly_list) {
+      // block-local list first: one global atomic per block, not per wave
       const unsigned long long mask = __ballot(poly);
       if (mask) {
         const uint32_t leader = (uint32_t)__ffsll((long long)mask) - 1u;
         uint32_t at = 0;
-        if (lane == leader) at = atomicAdd(poly_count, (uint32_t)__popcll(mask));
+        if (lane == leader) at = atomicAdd(&s_npoly, (uint32_t)__popcll(mask));
         at = __shfl(at, (int)leader);
-        if (poly) poly_list[at + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull))] = p;
+        if (poly) s_poly[at + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull))] = p;
       }
     }
   }
   __syncthreads();
   for (uint32_t k = threadIdx.x; k <= nb; k += BIN_BLOCK) hist[(uint64_t)k * nblk + blockIdx.x] = h[k];
+  if (poly_list) {
+    if (threadIdx.x == 0) s_at = s_npoly ? atomicAdd(poly_count, s_npoly) : 0u;
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < s_npoly; k += BIN_BLOCK) poly_list[s_at + k] = s_poly[k];
+  }
 }
 
 // exclusive scan of 256 values held one per thread; returns the thread's
@@ -257,39 +266,61 @@ __global__ __launch_bounds__(TILE) void k_bin_sums(GridConst g, DataConst d, Pro
   const bool want_phase = d.vec_rot == BRILLE_ROT_GAMMA && d.ec[1] > 0;
   const uint32_t n_atoms = d.ec[1] / 3u;
   const bool ph_in_lds = L == LAYOUT_GAMMA || (want_phase && n_atoms <= 4);
+  __shared__ uint32_t s_wave_tab[TILE / 64];
   if (Prop) __syncthreads();
-  {
-    PointRec<Real>& r = s_rec[tid];
-    r.nslot = 0;
-    if (p < n) {
-      const SortRec sr = srt[p];
-      s_out[tid] = sr.index;
-      Located P;
-      P.q[0] = sr.q[0]; P.q[1] = sr.q[1]; P.q[2] = sr.q[2];
-      P.tau[0] = P.tau[1] = P.tau[2] = 0;
-      P.node = sr.node;
-      P.rot = sr.meta & 255u;
-      P.invrot = (sr.meta >> 8) & 255u;
-      P.status = (int)meta_status(sr.meta);
-      P.tet = kNoTet;
-      P.nslot = 0;
-      P.nkept = 0;
-      const bool folded = P.status == BRILLE_PT_OK;
-      if (folded) {
-        if constexpr (Prop) {
-          bool verified = false;
-          node_slots(g, P, sr.hint, pc.margin, &verified);
-          if (sr.hint != kNoTet) {
-            atomicAdd(&s_tried, 1u);
-            if (verified) atomicAdd(&s_verified, 1u);
-          }
-        } else {
-          node_slots(g, P);
+  const bool live = p < n;
+  SortRec sr;
+  Located P;
+  P.status = BRILLE_PT_OK;
+  P.cube = kNoCube;
+  if (live) {
+    sr = srt[p];
+    P.q[0] = sr.q[0]; P.q[1] = sr.q[1]; P.q[2] = sr.q[2];
+    P.tau[0] = P.tau[1] = P.tau[2] = 0;
+    P.node = sr.node;
+    P.rot = sr.meta & 255u;
+    P.invrot = (sr.meta >> 8) & 255u;
+    P.status = (int)meta_status(sr.meta);
+    P.tet = kNoTet;
+    P.nslot = 0;
+    P.nkept = 0;
+    if (P.status == BRILLE_PT_OK) {
+      if constexpr (Prop) {
+        bool verified = false;
+        node_slots(g, P, sr.hint, pc.margin, &verified);
+        if (sr.hint != kNoTet) {
+          atomicAdd(&s_tried, 1u);
+          if (verified) atomicAdd(&s_verified, 1u);
         }
-        // failed fold-ins were reported by k_bin_fold
-        if (P.status != BRILLE_PT_OK) report_point(err, base + sr.index, P.status);
-        else store_record<Real, L, SC>(d, P, r, s_ph, s_q, tid, ph_in_lds, n_atoms);
+      } else {
+        node_slots(g, P);
       }
+      // failed fold-ins were reported by k_bin_fold
+      if (P.status != BRILLE_PT_OK) report_point(err, base + sr.index, P.status);
+    }
+  }
+  // Stable partition of the tile: points the cube factor table serves first,
+  // so that few waves of the sums mix the two kinds of point.
+  const bool tab = live && P.status == BRILLE_PT_OK && P.cube != kNoCube;
+  const uint32_t lane = tid & 63u, wave = tid >> 6;
+  const unsigned long long bal = __ballot(tab);
+  const uint32_t rank = (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
+  if (lane == 0) s_wave_tab[wave] = (uint32_t)__popcll(bal);
+  __syncthreads();
+  uint32_t before = 0, n_tab = 0;
+#pragma unroll
+  for (uint32_t w = 0; w < TILE / 64; ++w) {
+    const uint32_t c = s_wave_tab[w];
+    before += w < wave ? c : 0u;
+    n_tab += c;
+  }
+  const uint32_t slot = tab ? before + rank : n_tab + (tid - before - rank);
+  {
+    PointRec<Real>& r = s_rec[slot];
+    r.nslot = 0;
+    if (live) {
+      s_out[slot] = sr.index;
+      if (P.status == BRILLE_PT_OK) store_record<Real, L, SC>(d, P, r, s_ph, s_q, slot, ph_in_lds, n_atoms);
       write_aux(aux, sr.index, P);
     }
   }
@@ -299,7 +330,32 @@ __global__ __launch_bounds__(TILE) void k_bin_sums(GridConst g, DataConst d, Pro
     atomicAdd(&pc.stats[1], (unsigned long long)s_verified);
   }
   const uint32_t npts = n - tile0 < (uint32_t)TILE ? n - tile0 : (uint32_t)TILE;
-  tile_sums<Real, true, L, SC, SV>(g, d, s_rec, s_ph, s_q, s_out, 0, npts, ph_in_lds, vals, vecs, s_wide);
+  const bool table = (L == LAYOUT_GAMMA || L == LAYOUT_SCALAR) && cube_factors<Real>(d) != nullptr;
+  tile_sums<Real, true, L, SC, SV>(g, d, s_rec, s_ph, s_q, s_out, 0, npts, ph_in_lds, vals, vecs, s_wide, table);
+}
+
+// The cube factor table: thread t = (cube c, branch b) computes the eight
+// antiphase factors of c's rows against the row of its slot-0 vertex (the
+// pivot of every cube point with a non-zero slot-0 weight), with the sums'
+// own function, so a point summed from the table gets the same bits.
+template <typename Real, int SC>
+__global__ __launch_bounds__(256) void k_cube_factors(GridConst g, DataConst d, uint64_t n_units,
+                                                       typename Cplx<Real>::T* out) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_units) return;
+  const uint64_t c = t / d.B;
+  const uint32_t b = (uint32_t)(t - c * d.B);
+  const uint32_t* v = g.cube_v + 8 * c;
+  const Real* base = payload_rows<Real>(d);
+  const uint64_t stride = payload_stride<Real>(d);
+  Real pr[2 * SC];
+  load_row<Real, 2 * SC>(pr, base + ((uint64_t)v[7] * d.B + b) * stride);
+#pragma unroll
+  for (int x = 0; x < 8; ++x) {
+    Real row[2 * SC];
+    load_row<Real, 2 * SC>(row, base + ((uint64_t)v[7 - x] * d.B + b) * stride);
+    out[8 * t + x] = slot_antiphase<Real, SC>(pr, row);
+  }
 }
 
 using SumsKernel = void (*)(GridConst, DataConst, ProposalConst, const SortRec*, uint32_t, void*, void*,
@@ -333,6 +389,61 @@ static SumsKernel pick_sums(const DataConst& d, bool prop) {
   }
   if (wide_rows(d)) return k_bin_sums<Real, LAYOUT_WIDE, 0, 0, false>;
   return k_bin_sums<Real, LAYOUT_GENERIC, 0, 0, false>;
+}
+
+// SC of the one-pass sums kernel pick_sums selects, or 0 for the general ones
+static int onepass_sc(const DataConst& d) {
+  const int L = payload_layout(d);
+  if (d.sv != 1) return 0;
+  if (L == LAYOUT_GAMMA && (d.sc == 3 || d.sc == 6 || d.sc == 9 || d.sc == 12)) return (int)d.sc;
+  if (L == LAYOUT_SCALAR && d.sc >= 1 && d.sc <= 4) return (int)d.sc;
+  return 0;
+}
+
+template <typename Real>
+static int build_cube_factors(brille_grid* G, hipStream_t s, typename Cplx<Real>::T** out) {
+  using C = typename Cplx<Real>::T;
+  const DataConst& d = G->d;
+  const uint64_t units = G->n_cubes * d.B;
+  void* p = nullptr;
+  if (hipMalloc(&p, units * 8 * sizeof(C)) != hipSuccess)
+    return fail(BRILLE_ERR_DEVICE, "hipMalloc failed for the cube factor table");
+  G->data_allocs.push_back(p);
+  C* t = static_cast<C*>(p);
+  const dim3 grid((unsigned)((units + 255) / 256));
+  switch (onepass_sc(d)) {
+    case 1: hipLaunchKernelGGL((k_cube_factors<Real, 1>), grid, dim3(256), 0, s, G->g, d, units, t); break;
+    case 2: hipLaunchKernelGGL((k_cube_factors<Real, 2>), grid, dim3(256), 0, s, G->g, d, units, t); break;
+    case 3: hipLaunchKernelGGL((k_cube_factors<Real, 3>), grid, dim3(256), 0, s, G->g, d, units, t); break;
+    case 4: hipLaunchKernelGGL((k_cube_factors<Real, 4>), grid, dim3(256), 0, s, G->g, d, units, t); break;
+    case 6: hipLaunchKernelGGL((k_cube_factors<Real, 6>), grid, dim3(256), 0, s, G->g, d, units, t); break;
+    case 9: hipLaunchKernelGGL((k_cube_factors<Real, 9>), grid, dim3(256), 0, s, G->g, d, units, t); break;
+    case 12: hipLaunchKernelGGL((k_cube_factors<Real, 12>), grid, dim3(256), 0, s, G->g, d, units, t); break;
+    default: return fail(BRILLE_ERR_UNSUPPORTED, "no cube factor kernel for this payload");
+  }
+  if (hipGetLastError() != hipSuccess) return fail(BRILLE_ERR_DEVICE, "cube factor launch failed");
+  *out = t;
+  return BRILLE_OK;
+}
+
+// Builds the table of the given precision once per payload (freed with it).
+// Payloads whose sums are not the one-pass kind, and grids without cubes, go
+// without: their sums compute every factor.
+int ensure_cube_factors(brille_grid* G, int precision, hipStream_t s) {
+  DataConst& d = G->d;
+  if (G->n_cubes == 0 || d.B == 0 || onepass_sc(d) == 0) return BRILLE_OK;
+  if (precision == BRILLE_FP32) {
+    if (d.cube_ap32) return BRILLE_OK;
+    float2* t = nullptr;
+    const int rc = build_cube_factors<float>(G, s, &t);
+    if (rc == BRILLE_OK) d.cube_ap32 = t;
+    return rc;
+  }
+  if (d.cube_ap64) return BRILLE_OK;
+  double2* t = nullptr;
+  const int rc = build_cube_factors<double>(G, s, &t);
+  if (rc == BRILLE_OK) d.cube_ap64 = t;
+  return rc;
 }
 
 // the aux outputs of a pass that starts at point c0
@@ -383,6 +494,7 @@ void free_binned(brille_grid* G) {
 // stage_ms (nullable): adds the milliseconds of fold, bucket (scan + scatter)
 // and sums over all passes; timing synchronises after every pass
 int launch_binned(brille_grid* G, const LaunchArgs& a, int precision, hipStream_t s, float* stage_ms) {
+  if (int rc = ensure_cube_factors(G, precision, s); rc != BRILLE_OK) return rc;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   if (stage_ms)
     for (auto& e : ev)

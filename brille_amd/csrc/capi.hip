@@ -259,6 +259,7 @@ int brille_grid_create(const brille_grid_desc* desc, brille_grid** out) {
     return rc;
   }
   G->n_vertices = desc->n_vertices;
+  G->n_cubes = desc->n_cubes;
   G->n_polys = desc->n_polys;
   {
     int dev = 0;

@@ -88,6 +88,12 @@ struct DataConst {
   const double* rows64;
   const float* rows32;
   uint32_t stride64, stride32;  // in reals
+  // Antiphase factors of the cube nodes for points whose pivot is the cube's
+  // slot-0 vertex: (cube, branch, slot) -> e^{-i theta} of that slot's row
+  // against the slot-0 row, computed with the sums' own arithmetic (null until
+  // built; used by the bucketed sums only).
+  const float2* cube_ap32;
+  const double2* cube_ap64;
   // GammaTable
   uint32_t n_atoms, gamma_ops;
   const uint32_t* F0;

@@ -53,10 +53,12 @@ struct Located {
   uint32_t idx[8];
   double w[8];
   uint32_t pivot;  // first kept vertex: the antiphase reference of the sums
+  uint32_t cube;   // cube index when the pivot is slot 0 (kNoCube otherwise)
   int nslot;       // 0 (failed), 4 or 8
   int nkept;
   int status;
 };
+constexpr uint32_t kNoCube = 0xFFFFFFFFu;
 
 // What phase 2 needs of a point, in LDS.
 template <typename Real>
@@ -64,6 +66,7 @@ struct PointRec {
   uint32_t idx[8];
   Real w[8];
   uint32_t pivot;
+  uint32_t cube;  // cube whose slot-0 antiphase factors apply, or kNoCube
   uint16_t rot, invrot;
   uint8_t nslot, pad0, pad1, pad2;
 };
@@ -491,16 +494,40 @@ __device__ inline void unit_generic(const GridConst& g, const DataConst& d, cons
 // One pass: every row is read once, with 16-byte loads, and contributes its
 // antiphase and its weighted elements before the next row is read.
 
+template <typename Real>
+__device__ __forceinline__ const typename Cplx<Real>::T* cube_factors(const DataConst& d);
+template <>
+__device__ __forceinline__ const float2* cube_factors<float>(const DataConst& d) { return d.cube_ap32; }
+template <>
+__device__ __forceinline__ const double2* cube_factors<double>(const DataConst& d) { return d.cube_ap64; }
+
+// The antiphase of slot x's row against the pivot row p: Hermitian product and
+// closed form, one definition for the sums and for the cube factor table.
+template <typename Real, int SC>
+__device__ __forceinline__ typename Cplx<Real>::T slot_antiphase(const Real* p, const Real* x) {
+  Real re = 0, im = 0;
+#pragma unroll
+  for (int k = 0; k < SC; ++k) {
+    re += p[2 * k] * x[2 * k] + p[2 * k + 1] * x[2 * k + 1];
+    im += p[2 * k] * x[2 * k + 1] - p[2 * k + 1] * x[2 * k];
+  }
+  return antiphase_closed<Real>(re, im);
+}
+
+// table: use the cube factor table for points whose pivot is slot 0
 template <typename Real, int SC, int SV, bool GAMMA, int GRP = 0>
 __device__ __forceinline__ void unit_onepass(const GridConst& g, const DataConst& d, const PointRec<Real>& r,
                                              uint32_t b, Real* ov, typename Cplx<Real>::T* oc,
-                                             const typename Cplx<Real>::T* phases, bool cart) {
+                                             const typename Cplx<Real>::T* phases, bool cart, bool table = false) {
   using C = typename Cplx<Real>::T;
   constexpr int NR = 2 * SC + SV;
   const Real* base = payload_rows<Real>(d);
   const uint64_t stride = payload_stride<Real>(d);
+  // factors from the table: only cube points with the pivot in slot 0 have a row there
+  const bool tab = table && r.cube != kNoCube;
+  const C* fac = tab ? cube_factors<Real>(d) + ((uint64_t)r.cube * d.B + b) * 8 : nullptr;
   Real pr[2 * SC];
-  load_row<Real, 2 * SC>(pr, base + ((uint64_t)r.pivot * d.B + b) * stride);
+  if (!tab) load_row<Real, 2 * SC>(pr, base + ((uint64_t)r.pivot * d.B + b) * stride);
   Real acc[2 * SC];
   Real vacc[SV];
 #pragma unroll
@@ -517,17 +544,18 @@ __device__ __forceinline__ void unit_onepass(const GridConst& g, const DataConst
     Real row[GR][NR];
 #pragma unroll
     for (int x = 0; x < GR; ++x) load_row<Real, NR>(row[x], base + ((uint64_t)r.idx[x0 + x] * d.B + b) * stride);
+    C e[GR];
+    if (tab) {
+#pragma unroll
+      for (int x = 0; x < GR; ++x) e[x] = fac[x0 + x];
+    } else {
+#pragma unroll
+      for (int x = 0; x < GR; ++x) e[x] = slot_antiphase<Real, SC>(pr, row[x]);
+    }
 #pragma unroll
     for (int x = 0; x < GR; ++x) {
       const Real w = r.w[x0 + x];
-      Real re = 0, im = 0;
-#pragma unroll
-      for (int k = 0; k < SC; ++k) {
-        re += pr[2 * k] * row[x][2 * k] + pr[2 * k + 1] * row[x][2 * k + 1];
-        im += pr[2 * k] * row[x][2 * k + 1] - pr[2 * k + 1] * row[x][2 * k];
-      }
-      const C e = antiphase_closed<Real>(re, im);
-      const Real cx = w * e.x, cy = w * e.y;
+      const Real cx = w * e[x].x, cy = w * e[x].y;
 #pragma unroll
       for (int k = 0; k < SC; ++k) {
         acc[2 * k] += cx * row[x][2 * k] - cy * row[x][2 * k + 1];
@@ -726,6 +754,7 @@ __device__ inline void node_slots(const GridConst& g, Located& L, uint32_t hint 
   mat_vec(x, g.outer_to_xyz, L.q);  // brille::get_xyz
   const uint32_t node = L.node;
   L.tet = kNoTet;
+  L.cube = kNoCube;
   L.nslot = 0;
   L.nkept = 0;
   L.pivot = 0;
@@ -737,6 +766,7 @@ __device__ inline void node_slots(const GridConst& g, Located& L, uint32_t hint 
   if (g.node_type[node] == BRILLE_NODE_CUBE) {
     cube_slots(g, g.node_index[node], x, L.idx, L.w, tdef);
     L.nslot = 8;
+    if (L.w[0] != 0.0) L.cube = g.node_index[node];
   } else {
     bool ok = false;
     if (hint != kNoTet && tet_might_contain(g, hint, x, tdef) >= 0.0) {
@@ -820,6 +850,7 @@ __device__ __forceinline__ void store_record(const DataConst& d, const Located& 
     r.w[s] = (Real)P.w[s];
   }
   r.pivot = P.pivot;
+  r.cube = P.cube;
   r.rot = (uint16_t)P.rot;
   r.invrot = (uint16_t)P.invrot;
   r.nslot = (uint8_t)P.nslot;
@@ -842,7 +873,8 @@ template <typename Real, bool Fast, int L, int SC, int SV, int TB = TILE, int GR
 __device__ __forceinline__ void tile_sums(const GridConst& g, const DataConst& d, const PointRec<Real>* s_rec,
                                           const typename Cplx<Real>::T* s_ph, const double* s_q,
                                           const uint32_t* out_row, uint64_t row0, uint32_t npts, bool ph_in_lds,
-                                          void* vals_, void* vecs_, typename Cplx<Real>::T* s_wide = nullptr) {
+                                          void* vals_, void* vecs_, typename Cplx<Real>::T* s_wide = nullptr,
+                                          bool table = false) {
   using C = typename Cplx<Real>::T;
   constexpr int NPH = TileShape<L, SC>::NPH;
   const uint32_t B = d.B;
@@ -883,9 +915,9 @@ __device__ __forceinline__ void tile_sums(const GridConst& g, const DataConst& d
       continue;
     }
     if constexpr (L == LAYOUT_GAMMA) {
-      unit_onepass<Real, SC, SV, true, GRP>(g, d, r, b, ov, oc, s_ph + NPH * pl, cart);
+      unit_onepass<Real, SC, SV, true, GRP>(g, d, r, b, ov, oc, s_ph + NPH * pl, cart, table);
     } else if constexpr (L == LAYOUT_SCALAR) {
-      unit_onepass<Real, SC, SV, false, GRP>(g, d, r, b, ov, oc, nullptr, cart);
+      unit_onepass<Real, SC, SV, false, GRP>(g, d, r, b, ov, oc, nullptr, cart, table);
     } else {
       unit_generic<Real, !Fast>(g, d, r, s_q + 3 * pl, b, ov, oc, ph_in_lds ? s_ph + NPH * pl : nullptr);
     }

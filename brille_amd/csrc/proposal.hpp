@@ -91,29 +91,38 @@ __device__ __noinline__ void propose16(const ProposalConst& pc, const float* fea
   }
   uint32_t maxc = 0;
   for (uint32_t pi = 0; pi < n; ++pi) maxc = nc[pi] > maxc ? nc[pi] : maxc;
-  for (uint32_t s = 0; s < maxc; ++s) {
+  // candidates two at a time: the 8 row loads of a step are issued together
+  constexpr int CQ = 2;
+  for (uint32_t s0 = 0; s0 < maxc; s0 += CQ) {
+    bf16x8_t w[CQ][4];
+    float bias[CQ][4];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const bool act = s < ncr[r];
-      const uint32_t c = act ? t0r[r] + s : 0u;
-      float part = 0.f;
-      if (act) {
-        const bf16x8_t w = *reinterpret_cast<const bf16x8_t*>(pc.W3 + (size_t)c * kPropH + 8 * m);
+    for (int q = 0; q < CQ; ++q)
 #pragma unroll
-        for (int nb = 0; nb < 8; ++nb) part += (float)w[nb] * acc[nb][r];
+      for (int r = 0; r < 4; ++r) {
+        const uint32_t s = s0 + q;
+        const uint32_t c = s < ncr[r] ? t0r[r] + s : t0r[r];  // a valid row either way
+        w[q][r] = *reinterpret_cast<const bf16x8_t*>(pc.W3 + (size_t)c * kPropH + 8 * m);
+        bias[q][r] = pc.b3[c];
       }
-      part += __shfl_xor(part, 1);
-      part += __shfl_xor(part, 2);
-      part += __shfl_xor(part, 4);
-      part += __shfl_xor(part, 8);
-      if (act) {
-        const float z = part + pc.b3[c];
-        if (z > best[r]) {
+#pragma unroll
+    for (int q = 0; q < CQ; ++q)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const uint32_t s = s0 + q;
+        float part = 0.f;
+#pragma unroll
+        for (int nb = 0; nb < 8; ++nb) part += (float)w[q][r][nb] * acc[nb][r];
+        part += __shfl_xor(part, 1);
+        part += __shfl_xor(part, 2);
+        part += __shfl_xor(part, 4);
+        part += __shfl_xor(part, 8);
+        const float z = part + bias[q][r];
+        if (s < ncr[r] && z > best[r]) {
           best[r] = z;
-          bestc[r] = c;
+          bestc[r] = t0r[r] + s;
         }
       }
-    }
   }
   if (m == 0) {
 #pragma unroll
