@@ -90,7 +90,7 @@ __device__ __forceinline__ void batch_tile(const BatchGrid& bg, const BatchSlot&
     Located P;
     locate<true>(bg.g, Qp, fold, P);
     if (P.status != BRILLE_PT_OK) report_point(bg.err, p, P.status);
-    else store_record<Real, L, SC>(bg.d, P, r, s_ph, s_q, tid, bg.ph_in_lds != 0, bg.n_atoms);
+    else store_record<Real, L, SC>(bg.g, bg.d, P, r, s_ph, s_q, tid, bg.ph_in_lds != 0, bg.n_atoms);
   }
   __syncthreads();
   // four rows in flight: the persistent kernel's registers are the largest of all its branches
@@ -105,7 +105,8 @@ __global__ __launch_bounds__(TB) void k_batch(const BatchGrid* grids, const Batc
   using C = typename Cplx<Real>::T;
   __shared__ PointRec<Real> s_rec[TB];
   __shared__ C s_ph[TB * 4];
-  __shared__ double s_q[3 * TB];
+  // q (general and wide layouts) or the Gamma frames
+  __shared__ double s_q[(3 > frame_words<Real>() ? 3 : frame_words<Real>()) * TB];
   __shared__ C s_wide[(TB / 64) * kWideMax];
   __shared__ unsigned long long s_t;
   const uint64_t n_tiles = hdr->n_tiles;

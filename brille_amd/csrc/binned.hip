@@ -249,7 +249,7 @@ __global__ __launch_bounds__(TILE) void k_bin_sums(GridConst g, DataConst d, Pro
                                                     uint32_t n, void* vals, void* vecs, brille_aux_out aux,
                                                     ErrRec* err, uint64_t base) {
   using C = typename Cplx<Real>::T;
-  using S = TileShape<L, SC>;
+  using S = TileShape<L, SC, TILE, Real>;
   __shared__ PointRec<Real> s_rec[TILE];
   __shared__ C s_ph[TILE * S::NPH];
   __shared__ double s_q[S::NQ];
@@ -301,7 +301,8 @@ __global__ __launch_bounds__(TILE) void k_bin_sums(GridConst g, DataConst d, Pro
   }
   // Stable partition of the tile: points the cube factor table serves first,
   // so that few waves of the sums mix the two kinds of point.
-  const bool tab = live && P.status == BRILLE_PT_OK && P.cube != kNoCube;
+  const bool table = (L == LAYOUT_GAMMA || L == LAYOUT_SCALAR) && cube_rows<Real>(d) != nullptr;
+  const bool tab = table && live && P.status == BRILLE_PT_OK && P.cube != kNoCube;
   const uint32_t lane = tid & 63u, wave = tid >> 6;
   const unsigned long long bal = __ballot(tab);
   const uint32_t rank = (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
@@ -320,7 +321,7 @@ __global__ __launch_bounds__(TILE) void k_bin_sums(GridConst g, DataConst d, Pro
     r.nslot = 0;
     if (live) {
       s_out[slot] = sr.index;
-      if (P.status == BRILLE_PT_OK) store_record<Real, L, SC>(d, P, r, s_ph, s_q, slot, ph_in_lds, n_atoms);
+      if (P.status == BRILLE_PT_OK) store_record<Real, L, SC>(g, d, P, r, s_ph, s_q, slot, ph_in_lds, n_atoms);
       write_aux(aux, sr.index, P);
     }
   }
@@ -330,32 +331,36 @@ __global__ __launch_bounds__(TILE) void k_bin_sums(GridConst g, DataConst d, Pro
     atomicAdd(&pc.stats[1], (unsigned long long)s_verified);
   }
   const uint32_t npts = n - tile0 < (uint32_t)TILE ? n - tile0 : (uint32_t)TILE;
-  const bool table = (L == LAYOUT_GAMMA || L == LAYOUT_SCALAR) && cube_factors<Real>(d) != nullptr;
-  tile_sums<Real, true, L, SC, SV>(g, d, s_rec, s_ph, s_q, s_out, 0, npts, ph_in_lds, vals, vecs, s_wide, table);
+  // the one-lane-per-unit sums see only the points the table does not serve:
+  // fewer rows in flight keep the registers for occupancy
+  constexpr int GRP = sizeof(Real) == 4 ? 2 : 1;
+  tile_sums<Real, true, L, SC, SV, TILE, GRP>(g, d, s_rec, s_ph, s_q, s_out, 0, npts, ph_in_lds, vals, vecs, s_wide,
+                                              table, n_tab);
 }
 
-// The cube factor table: thread t = (cube c, branch b) computes the eight
-// antiphase factors of c's rows against the row of its slot-0 vertex (the
-// pivot of every cube point with a non-zero slot-0 weight), with the sums'
-// own function, so a point summed from the table gets the same bits.
+// The cube row table: thread t = ((cube c, slot x), mode b) phases the row of
+// c's slot-x vertex against the row of its slot-0 vertex with the sums' own
+// functions, so a point summed from the table gets the same bits.
 template <typename Real, int SC>
-__global__ __launch_bounds__(256) void k_cube_factors(GridConst g, DataConst d, uint64_t n_units,
-                                                       typename Cplx<Real>::T* out) {
+__global__ __launch_bounds__(256) void k_cube_rows(GridConst g, DataConst d, uint64_t n_units, Real* out) {
   const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n_units) return;
-  const uint64_t c = t / d.B;
-  const uint32_t b = (uint32_t)(t - c * d.B);
+  const uint64_t cx = t / d.B;
+  const uint32_t b = (uint32_t)(t - cx * d.B);
+  const uint64_t c = cx >> 3;
+  const int x = (int)(cx & 7u);
   const uint32_t* v = g.cube_v + 8 * c;
   const Real* base = payload_rows<Real>(d);
   const uint64_t stride = payload_stride<Real>(d);
   Real pr[2 * SC];
   load_row<Real, 2 * SC>(pr, base + ((uint64_t)v[7] * d.B + b) * stride);
-#pragma unroll
-  for (int x = 0; x < 8; ++x) {
-    Real row[2 * SC];
-    load_row<Real, 2 * SC>(row, base + ((uint64_t)v[7 - x] * d.B + b) * stride);
-    out[8 * t + x] = slot_antiphase<Real, SC>(pr, row);
-  }
+  const Real* src = base + ((uint64_t)v[7 - x] * d.B + b) * stride;
+  Real row[2 * SC];
+  load_row<Real, 2 * SC>(row, src);
+  const typename Cplx<Real>::T e = slot_antiphase<Real, SC>(pr, row);
+  Real* dst = out + t * stride;
+  for (int k = 0; k < SC; ++k) phase_elem<Real>(e, row[2 * k], row[2 * k + 1], &dst[2 * k], &dst[2 * k + 1]);
+  for (uint32_t k = 2 * SC; k < stride; ++k) dst[k] = src[k];  // values and zero padding
 }
 
 using SumsKernel = void (*)(GridConst, DataConst, ProposalConst, const SortRec*, uint32_t, void*, void*,
@@ -401,48 +406,56 @@ static int onepass_sc(const DataConst& d) {
 }
 
 template <typename Real>
-static int build_cube_factors(brille_grid* G, hipStream_t s, typename Cplx<Real>::T** out) {
-  using C = typename Cplx<Real>::T;
+static uint64_t payload_stride_host(const DataConst& d) {
+  return sizeof(Real) == 8 ? d.stride64 : d.stride32;
+}
+
+template <typename Real>
+static int build_cube_rows(brille_grid* G, hipStream_t s, Real** out) {
   const DataConst& d = G->d;
-  const uint64_t units = G->n_cubes * d.B;
+  const uint64_t units = G->n_cubes * 8 * d.B;
   void* p = nullptr;
-  if (hipMalloc(&p, units * 8 * sizeof(C)) != hipSuccess)
-    return fail(BRILLE_ERR_DEVICE, "hipMalloc failed for the cube factor table");
+  if (hipMalloc(&p, units * payload_stride_host<Real>(d) * sizeof(Real)) != hipSuccess)
+    return fail(BRILLE_ERR_DEVICE, "hipMalloc failed for the cube row table");
   G->data_allocs.push_back(p);
-  C* t = static_cast<C*>(p);
+  Real* t = static_cast<Real*>(p);
   const dim3 grid((unsigned)((units + 255) / 256));
   switch (onepass_sc(d)) {
-    case 1: hipLaunchKernelGGL((k_cube_factors<Real, 1>), grid, dim3(256), 0, s, G->g, d, units, t); break;
-    case 2: hipLaunchKernelGGL((k_cube_factors<Real, 2>), grid, dim3(256), 0, s, G->g, d, units, t); break;
-    case 3: hipLaunchKernelGGL((k_cube_factors<Real, 3>), grid, dim3(256), 0, s, G->g, d, units, t); break;
-    case 4: hipLaunchKernelGGL((k_cube_factors<Real, 4>), grid, dim3(256), 0, s, G->g, d, units, t); break;
-    case 6: hipLaunchKernelGGL((k_cube_factors<Real, 6>), grid, dim3(256), 0, s, G->g, d, units, t); break;
-    case 9: hipLaunchKernelGGL((k_cube_factors<Real, 9>), grid, dim3(256), 0, s, G->g, d, units, t); break;
-    case 12: hipLaunchKernelGGL((k_cube_factors<Real, 12>), grid, dim3(256), 0, s, G->g, d, units, t); break;
-    default: return fail(BRILLE_ERR_UNSUPPORTED, "no cube factor kernel for this payload");
+    case 1: hipLaunchKernelGGL((k_cube_rows<Real, 1>), grid, dim3(256), 0, s, G->g, d, units, t); break;
+    case 2: hipLaunchKernelGGL((k_cube_rows<Real, 2>), grid, dim3(256), 0, s, G->g, d, units, t); break;
+    case 3: hipLaunchKernelGGL((k_cube_rows<Real, 3>), grid, dim3(256), 0, s, G->g, d, units, t); break;
+    case 4: hipLaunchKernelGGL((k_cube_rows<Real, 4>), grid, dim3(256), 0, s, G->g, d, units, t); break;
+    case 6: hipLaunchKernelGGL((k_cube_rows<Real, 6>), grid, dim3(256), 0, s, G->g, d, units, t); break;
+    case 9: hipLaunchKernelGGL((k_cube_rows<Real, 9>), grid, dim3(256), 0, s, G->g, d, units, t); break;
+    case 12: hipLaunchKernelGGL((k_cube_rows<Real, 12>), grid, dim3(256), 0, s, G->g, d, units, t); break;
+    default: return fail(BRILLE_ERR_UNSUPPORTED, "no cube row kernel for this payload");
   }
-  if (hipGetLastError() != hipSuccess) return fail(BRILLE_ERR_DEVICE, "cube factor launch failed");
+  if (hipGetLastError() != hipSuccess) return fail(BRILLE_ERR_DEVICE, "cube row table launch failed");
   *out = t;
   return BRILLE_OK;
 }
 
-// Builds the table of the given precision once per payload (freed with it).
-// Payloads whose sums are not the one-pass kind, and grids without cubes, go
-// without: their sums compute every factor.
-int ensure_cube_factors(brille_grid* G, int precision, hipStream_t s) {
+// Builds the table of the given precision once per payload (freed with it),
+// eight rows per cube: payloads whose sums are not the one-pass kind, grids
+// without cubes and tables beyond kCubeRowsMax go without, and their sums
+// phase every row on the fly.
+constexpr uint64_t kCubeRowsMax = (4ull << 30) - 1;  // the sums address the table with 32-bit offsets
+int ensure_cube_rows(brille_grid* G, int precision, hipStream_t s) {
   DataConst& d = G->d;
   if (G->n_cubes == 0 || d.B == 0 || onepass_sc(d) == 0) return BRILLE_OK;
+  const uint64_t row_bytes = precision == BRILLE_FP32 ? d.stride32 * 4ull : d.stride64 * 8ull;
+  if (G->n_cubes * 8 * d.B * row_bytes > kCubeRowsMax) return BRILLE_OK;
   if (precision == BRILLE_FP32) {
-    if (d.cube_ap32) return BRILLE_OK;
-    float2* t = nullptr;
-    const int rc = build_cube_factors<float>(G, s, &t);
-    if (rc == BRILLE_OK) d.cube_ap32 = t;
+    if (d.cube_rows32) return BRILLE_OK;
+    float* t = nullptr;
+    const int rc = build_cube_rows<float>(G, s, &t);
+    if (rc == BRILLE_OK) d.cube_rows32 = t;
     return rc;
   }
-  if (d.cube_ap64) return BRILLE_OK;
-  double2* t = nullptr;
-  const int rc = build_cube_factors<double>(G, s, &t);
-  if (rc == BRILLE_OK) d.cube_ap64 = t;
+  if (d.cube_rows64) return BRILLE_OK;
+  double* t = nullptr;
+  const int rc = build_cube_rows<double>(G, s, &t);
+  if (rc == BRILLE_OK) d.cube_rows64 = t;
   return rc;
 }
 
@@ -494,7 +507,7 @@ void free_binned(brille_grid* G) {
 // stage_ms (nullable): adds the milliseconds of fold, bucket (scan + scatter)
 // and sums over all passes; timing synchronises after every pass
 int launch_binned(brille_grid* G, const LaunchArgs& a, int precision, hipStream_t s, float* stage_ms) {
-  if (int rc = ensure_cube_factors(G, precision, s); rc != BRILLE_OK) return rc;
+  if (int rc = ensure_cube_rows(G, precision, s); rc != BRILLE_OK) return rc;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   if (stage_ms)
     for (auto& e : ev)

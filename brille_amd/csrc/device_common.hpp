@@ -88,12 +88,13 @@ struct DataConst {
   const double* rows64;
   const float* rows32;
   uint32_t stride64, stride32;  // in reals
-  // Antiphase factors of the cube nodes for points whose pivot is the cube's
-  // slot-0 vertex: (cube, branch, slot) -> e^{-i theta} of that slot's row
-  // against the slot-0 row, computed with the sums' own arithmetic (null until
-  // built; used by the bucketed sums only).
-  const float2* cube_ap32;
-  const double2* cube_ap64;
+  // Cube row table (null until built; read by the bucketed sums only): for
+  // cube c, slot x and mode b, at ((c * 8 + x) * B + b) * stride, the row of
+  // the cube's slot-x vertex with every vector element multiplied by its
+  // antiphase against the slot-0 vertex (the pivot of every cube point whose
+  // slot-0 weight is non-zero), values as they are; same stride as the rows.
+  const float* cube_rows32;
+  const double* cube_rows64;
   // GammaTable
   uint32_t n_atoms, gamma_ops;
   const uint32_t* F0;

@@ -54,8 +54,8 @@ bool wide_rows(const DataConst& d);
 bool binned_applies(const brille_grid* G, const LaunchArgs& a, int kernel);
 int launch_binned(brille_grid* G, const LaunchArgs& a, int precision, hipStream_t s, float* stage_ms);
 void free_binned(brille_grid* G);
-// the cube antiphase table of the bucketed sums, built on first use (data_allocs)
-int ensure_cube_factors(brille_grid* G, int precision, hipStream_t s);
+// the cube row table of the bucketed sums, built on first use (data_allocs)
+int ensure_cube_rows(brille_grid* G, int precision, hipStream_t s);
 // the first failing point of a call as the reference's error text
 int report_point_error(const ErrRec& e, const double* Q, uint64_t n, bool host_mem);
 // fold: FOLD_IR (ir_interpolate_at) or FOLD_FIRST_ZONE (interpolate_at); opts->no_move wins

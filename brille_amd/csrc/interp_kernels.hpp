@@ -71,6 +71,23 @@ struct PointRec {
   uint8_t nslot, pad0, pad1, pad2;
 };
 
+// Per-point constants of the Gamma rotation (rip_gamma_complex,
+// src/interpolator_gamma.tpp:100-114), kept in LDS by phase 1 so that the sums
+// read no tables: R is invRidx (integer, or Cartesian for angstrom units) as
+// the sums' scale factors, to[a] the output atom of source atom a (GammaTable
+// F0) and from[o] the source atom written to output atom o (255: none; the
+// last source atom wins, as in a loop over source atoms).
+template <typename Real>
+struct GammaFrame {
+  Real R[9];
+  uint8_t to[4];
+  uint8_t from[4];
+};
+template <typename Real>
+constexpr int frame_words() {
+  return (int)((sizeof(GammaFrame<Real>) + 7) / 8);
+}
+
 // ---------------------------------------------------------------------------
 // payload rows
 
@@ -372,6 +389,39 @@ __device__ inline typename Cplx<Real>::T gamma_phase(const DataConst& d, const d
 // phase 2, general payload: any elements, rotations, GammaTable and
 // PermutationTable; two passes over the rows (antiphases, then sums).
 
+__device__ __forceinline__ float fma_r(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fma_r(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+// complex element k of a row times its antiphase e (the values are not phased)
+template <typename Real>
+__device__ __forceinline__ void phase_elem(typename Cplx<Real>::T e, Real a, Real b, Real* px, Real* py) {
+  *px = e.x * a - e.y * b;
+  *py = e.x * b + e.y * a;
+}
+
+// One weighted term of a sum.  Literal (the plain kernel): acc + (w e) v, as
+// the reference writes it; fast: the antiphased element e v rounded, then
+// weighted with one fused multiply-add, which is unit_onepass's arithmetic.
+template <typename Real, bool Literal>
+__device__ __forceinline__ Real wsum_real(Real acc, Real w, Real v) {
+  if constexpr (Literal) return acc + w * v;
+  else return fma_r(w, v, acc);
+}
+// cw: w e (literal) or e (fast)
+template <typename Real, bool Literal>
+__device__ __forceinline__ typename Cplx<Real>::T wsum_cplx(typename Cplx<Real>::T acc, Real w,
+                                                            typename Cplx<Real>::T cw, typename Cplx<Real>::T v) {
+  if constexpr (Literal) {
+    return Elem<typename Cplx<Real>::T>::add(acc, cmul(cw, v));
+  } else {
+    Real px, py;
+    phase_elem<Real>(cw, v.x, v.y, &px, &py);
+    acc.x = fma_r(w, px, acc.x);
+    acc.y = fma_r(w, py, acc.y);
+    return acc;
+  }
+}
+
 template <typename Real, bool Literal>
 __device__ inline void unit_generic(const GridConst& g, const DataConst& d, const PointRec<Real>& r, const double* q,
                                     uint32_t b, Real* ov, typename Cplx<Real>::T* oc,
@@ -394,8 +444,12 @@ __device__ inline void unit_generic(const GridConst& g, const DataConst& d, cons
       Real re, im;
       hermitian<Real>(prow, row[x], sc, &re, &im);
       C e = antiphase_of<Real, Literal>(re, im);
-      cw[x].x = w[x] * e.x;  // weights[x]*eith
-      cw[x].y = w[x] * e.y;
+      if constexpr (Literal) {
+        cw[x].x = w[x] * e.x;  // weights[x]*eith
+        cw[x].y = w[x] * e.y;
+      } else {
+        cw[x] = e;  // the fast sums weight the antiphased element (unit_onepass's arithmetic)
+      }
     }
   }
   // values  (interpolate_at_mix, real part)
@@ -404,7 +458,7 @@ __device__ inline void unit_generic(const GridConst& g, const DataConst& d, cons
     Real acc = 0;
 #pragma unroll
     for (int x = 0; x < 8; ++x)
-      if (w[x] != (Real)0) acc += w[x] * row[x][2 * sc + e];
+      if (w[x] != (Real)0) acc = wsum_real<Real, Literal>(acc, w[x], row[x][2 * sc + e]);
     ov[e] = acc;
   }
   for (uint32_t k = 0; k < nv; ++k, e += 3) {
@@ -413,7 +467,7 @@ __device__ inline void unit_generic(const GridConst& g, const DataConst& d, cons
     for (int x = 0; x < 8; ++x)
       if (w[x] != (Real)0)
 #pragma unroll
-        for (int j = 0; j < 3; ++j) v[j] += w[x] * row[x][2 * sc + e + j];
+        for (int j = 0; j < 3; ++j) v[j] = wsum_real<Real, Literal>(v[j], w[x], row[x][2 * sc + e + j]);
     rotate_vec3<Real, Real>(d.val_rot, d.val_lu, g, r.rot, r.invrot, v);
     ov[e] = v[0]; ov[e + 1] = v[1]; ov[e + 2] = v[2];
   }
@@ -425,7 +479,7 @@ __device__ inline void unit_generic(const GridConst& g, const DataConst& d, cons
     for (int x = 0; x < 8; ++x)
       if (w[x] != (Real)0)
 #pragma unroll
-        for (int j = 0; j < 9; ++j) m[j] += w[x] * row[x][2 * sc + e + j];
+        for (int j = 0; j < 9; ++j) m[j] = wsum_real<Real, Literal>(m[j], w[x], row[x][2 * sc + e + j]);
     rotate_mat3<Real, Real>(d.val_rot, d.val_lu, g, r.rot, r.invrot, m);
 #pragma unroll
     for (int j = 0; j < 9; ++j) ov[e + j] = m[j];
@@ -443,7 +497,7 @@ __device__ inline void unit_generic(const GridConst& g, const DataConst& d, cons
     C acc = Elem<C>::zero();
 #pragma unroll
     for (int x = 0; x < 8; ++x)
-      if (w[x] != (Real)0) acc = Elem<C>::add(acc, cmul(cw[x], vec_at(x, e)));
+      if (w[x] != (Real)0) acc = wsum_cplx<Real, Literal>(acc, w[x], cw[x], vec_at(x, e));
     oc[e] = acc;
   }
   const bool gamma = d.vec_rot == BRILLE_ROT_GAMMA;
@@ -454,7 +508,7 @@ __device__ inline void unit_generic(const GridConst& g, const DataConst& d, cons
     for (int x = 0; x < 8; ++x)
       if (w[x] != (Real)0)
 #pragma unroll
-        for (int j = 0; j < 3; ++j) v[j] = Elem<C>::add(v[j], cmul(cw[x], vec_at(x, e + j)));
+        for (int j = 0; j < 3; ++j) v[j] = wsum_cplx<Real, Literal>(v[j], w[x], cw[x], vec_at(x, e + j));
     if (gamma) {
       // rip_gamma_complex  src/interpolator_gamma.tpp:100-114
       C t[3];
@@ -478,7 +532,7 @@ __device__ inline void unit_generic(const GridConst& g, const DataConst& d, cons
     for (int x = 0; x < 8; ++x)
       if (w[x] != (Real)0)
 #pragma unroll
-        for (int j = 0; j < 9; ++j) m[j] = Elem<C>::add(m[j], cmul(cw[x], vec_at(x, e + j)));
+        for (int j = 0; j < 9; ++j) m[j] = wsum_cplx<Real, Literal>(m[j], w[x], cw[x], vec_at(x, e + j));
     rotate_mat3<C, Real>(d.vec_rot, d.vec_lu, g, r.rot, r.invrot, m);
 #pragma unroll
     for (int j = 0; j < 9; ++j) oc[e + j] = m[j];
@@ -495,14 +549,20 @@ __device__ inline void unit_generic(const GridConst& g, const DataConst& d, cons
 // antiphase and its weighted elements before the next row is read.
 
 template <typename Real>
-__device__ __forceinline__ const typename Cplx<Real>::T* cube_factors(const DataConst& d);
+__device__ __forceinline__ const Real* cube_rows(const DataConst& d);
 template <>
-__device__ __forceinline__ const float2* cube_factors<float>(const DataConst& d) { return d.cube_ap32; }
+__device__ __forceinline__ const float* cube_rows<float>(const DataConst& d) { return d.cube_rows32; }
 template <>
-__device__ __forceinline__ const double2* cube_factors<double>(const DataConst& d) { return d.cube_ap64; }
+__device__ __forceinline__ const double* cube_rows<double>(const DataConst& d) { return d.cube_rows64; }
+
+// antiphased row of slot x of cube c for mode b in the cube row table
+template <typename Real>
+__device__ __forceinline__ const Real* cube_row_at(const DataConst& d, uint32_t c, int x, uint32_t b) {
+  return cube_rows<Real>(d) + (((uint64_t)c * 8 + (uint64_t)x) * d.B + b) * payload_stride<Real>(d);
+}
 
 // The antiphase of slot x's row against the pivot row p: Hermitian product and
-// closed form, one definition for the sums and for the cube factor table.
+// closed form, one definition for the sums and for the cube row table.
 template <typename Real, int SC>
 __device__ __forceinline__ typename Cplx<Real>::T slot_antiphase(const Real* p, const Real* x) {
   Real re = 0, im = 0;
@@ -514,18 +574,21 @@ __device__ __forceinline__ typename Cplx<Real>::T slot_antiphase(const Real* p, 
   return antiphase_closed<Real>(re, im);
 }
 
-// table: use the cube factor table for points whose pivot is slot 0
+// Every one-pass sum is  acc += w * (e (x) row)  element by element, the
+// product rounded first and the weight applied with one fused multiply-add:
+// the cube row table stores the rounded products, so points summed from the
+// table and points summed on the fly get the same bits.
+// table: antiphased rows from the cube row table for the points it serves
 template <typename Real, int SC, int SV, bool GAMMA, int GRP = 0>
-__device__ __forceinline__ void unit_onepass(const GridConst& g, const DataConst& d, const PointRec<Real>& r,
-                                             uint32_t b, Real* ov, typename Cplx<Real>::T* oc,
-                                             const typename Cplx<Real>::T* phases, bool cart, bool table = false) {
+__device__ __forceinline__ void unit_onepass(const DataConst& d, const PointRec<Real>& r, uint32_t b, Real* ov,
+                                             typename Cplx<Real>::T* oc, const typename Cplx<Real>::T* phases,
+                                             const GammaFrame<Real>* fr, bool table = false) {
   using C = typename Cplx<Real>::T;
   constexpr int NR = 2 * SC + SV;
   const Real* base = payload_rows<Real>(d);
   const uint64_t stride = payload_stride<Real>(d);
-  // factors from the table: only cube points with the pivot in slot 0 have a row there
+  // only cube points with the pivot in slot 0 have rows in the table
   const bool tab = table && r.cube != kNoCube;
-  const C* fac = tab ? cube_factors<Real>(d) + ((uint64_t)r.cube * d.B + b) * 8 : nullptr;
   Real pr[2 * SC];
   if (!tab) load_row<Real, 2 * SC>(pr, base + ((uint64_t)r.pivot * d.B + b) * stride);
   Real acc[2 * SC];
@@ -542,34 +605,32 @@ __device__ __forceinline__ void unit_onepass(const GridConst& g, const DataConst
 #pragma unroll
   for (int x0 = 0; x0 < 8; x0 += GR) {
     Real row[GR][NR];
-#pragma unroll
-    for (int x = 0; x < GR; ++x) load_row<Real, NR>(row[x], base + ((uint64_t)r.idx[x0 + x] * d.B + b) * stride);
-    C e[GR];
     if (tab) {
 #pragma unroll
-      for (int x = 0; x < GR; ++x) e[x] = fac[x0 + x];
+      for (int x = 0; x < GR; ++x) load_row<Real, NR>(row[x], cube_row_at<Real>(d, r.cube, x0 + x, b));
     } else {
 #pragma unroll
-      for (int x = 0; x < GR; ++x) e[x] = slot_antiphase<Real, SC>(pr, row[x]);
+      for (int x = 0; x < GR; ++x) load_row<Real, NR>(row[x], base + ((uint64_t)r.idx[x0 + x] * d.B + b) * stride);
+#pragma unroll
+      for (int x = 0; x < GR; ++x) {
+        const C e = slot_antiphase<Real, SC>(pr, row[x]);
+#pragma unroll
+        for (int k = 0; k < SC; ++k)
+          phase_elem<Real>(e, row[x][2 * k], row[x][2 * k + 1], &row[x][2 * k], &row[x][2 * k + 1]);
+      }
     }
 #pragma unroll
     for (int x = 0; x < GR; ++x) {
       const Real w = r.w[x0 + x];
-      const Real cx = w * e[x].x, cy = w * e[x].y;
 #pragma unroll
-      for (int k = 0; k < SC; ++k) {
-        acc[2 * k] += cx * row[x][2 * k] - cy * row[x][2 * k + 1];
-        acc[2 * k + 1] += cx * row[x][2 * k + 1] + cy * row[x][2 * k];
-      }
+      for (int k = 0; k < 2 * SC; ++k) acc[k] = fma_r(w, row[x][k], acc[k]);
 #pragma unroll
-      for (int s = 0; s < SV; ++s) vacc[s] += w * row[x][2 * SC + s];
+      for (int s = 0; s < SV; ++s) vacc[s] = fma_r(w, row[x][2 * SC + s], vacc[s]);
     }
   }
 #pragma unroll
   for (int s = 0; s < SV; ++s) ov[s] = vacc[s];
   if constexpr (GAMMA) {
-    const int32_t* iR = g.rot + 9 * (uint64_t)r.invrot;
-    const double* cR = cart ? d.cart_rot + 9 * (uint64_t)r.invrot : nullptr;
 #pragma unroll
     for (int a = 0; a < SC / 3; ++a) {
       C v[3], t[3];
@@ -578,9 +639,8 @@ __device__ __forceinline__ void unit_onepass(const GridConst& g, const DataConst
         v[j].x = acc[2 * (3 * a + j)];
         v[j].y = acc[2 * (3 * a + j) + 1];
       }
-      if (cart) rot3<C, double, Real>(t, cR, v);
-      else rot3<C, int32_t, Real>(t, iR, v);
-      const uint32_t l = d.F0[a * d.gamma_ops + r.invrot];
+      rot3<C, Real, Real>(t, fr->R, v);
+      const uint32_t l = fr->to[a];
       const C ph = phases[a];
       C* dst = oc + 3u * l;
 #pragma unroll
@@ -589,6 +649,135 @@ __device__ __forceinline__ void unit_onepass(const GridConst& g, const DataConst
   } else {
 #pragma unroll
     for (int k = 0; k < SC; ++k) oc[k] = C{acc[2 * k], acc[2 * k + 1]};
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Cooperative one-pass sums for the points the cube row table serves: G lanes
+// per (point, mode) unit, lane q reading the 16-byte chunk q of each of the
+// unit's eight antiphased rows.  A cube's rows for one slot lie mode after
+// mode, so one wave load brings whole rows from a few cache lines instead of a
+// 16-byte piece of each of 64 rows, and the sum is one fused multiply-add per
+// element and slot.  Every output element gets unit_onepass's arithmetic in
+// the same order, so both give the same bits.
+
+// lanes per unit: the row stride in 16-byte chunks (1 = no cooperation)
+template <typename Real, int SC, int SV>
+constexpr int coop_lanes() {
+  const int bytes = ((2 * SC + SV) * (int)sizeof(Real) + 15) / 16 * 16;
+  if (bytes > 128) return 1;
+  int p = 16;
+  while (p < bytes) p <<= 1;
+  return p / 16;
+}
+
+// the value of lane SRC of this lane's aligned group of G lanes
+template <int G, int SRC>
+__device__ __forceinline__ int grp_bcast_i(int v) {
+  static_assert(G == 2 || G == 4 || G == 8, "groups of 2, 4 or 8 lanes");
+  static_assert(SRC >= 0 && SRC < G, "source lane inside the group");
+  if constexpr (G == 4) {
+    return __builtin_amdgcn_mov_dpp(v, SRC * 0x55, 0xF, 0xF, false);
+  } else if constexpr (G == 2) {
+    return __builtin_amdgcn_mov_dpp(v, SRC | (SRC << 2) | ((2 + SRC) << 4) | ((2 + SRC) << 6), 0xF, 0xF, false);
+  } else {
+    // ds_swizzle bit mode: source lane = (lane & 0x18) | SRC within each half wave
+    return __builtin_amdgcn_ds_swizzle(v, 0x18 | (SRC << 5));
+  }
+}
+template <int G, int SRC>
+__device__ __forceinline__ float grp_bcast(float v) {
+  return __int_as_float(grp_bcast_i<G, SRC>(__float_as_int(v)));
+}
+template <int G, int SRC>
+__device__ __forceinline__ double grp_bcast(double v) {
+  const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+  const int lo = grp_bcast_i<G, SRC>((int)(uint32_t)u);
+  const int hi = grp_bcast_i<G, SRC>((int)(uint32_t)(u >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)(uint32_t)hi << 32) | (uint32_t)lo));
+}
+
+// complex element M of the group's accumulator (lane M / CPL, pair M % CPL)
+template <typename Real, int G, int CPL, int M>
+__device__ __forceinline__ typename Cplx<Real>::T grp_elem(const Real* acc) {
+  typename Cplx<Real>::T e;
+  e.x = grp_bcast<G, (M / CPL) % G>(acc[2 * (M % CPL)]);
+  e.y = grp_bcast<G, (M / CPL) % G>(acc[2 * (M % CPL) + 1]);
+  return e;
+}
+template <typename Real, int G, int CPL, int SC, int M = 0>
+__device__ __forceinline__ void grp_gather(const Real* acc, typename Cplx<Real>::T* all) {
+  if constexpr (M < SC) {
+    all[M] = grp_elem<Real, G, CPL, M>(acc);
+    grp_gather<Real, G, CPL, SC, M + 1>(acc, all);
+  }
+}
+
+// off: byte offset of chunk q of the unit's slot-0 row in the table (below
+// 4 GiB), slot_bytes: bytes from one slot's rows to the next (8 B rows)
+template <typename Real, int SC, int SV, bool GAMMA, int G>
+__device__ __forceinline__ void unit_coop(const DataConst& d, const PointRec<Real>& r, uint32_t off,
+                                          uint32_t slot_bytes, uint32_t q, Real* ov, typename Cplx<Real>::T* oc,
+                                          const typename Cplx<Real>::T* phases, const GammaFrame<Real>* fr) {
+  using C = typename Cplx<Real>::T;
+  using V = typename Cplx<Real>::V16;
+  constexpr int CPL = 8 / (int)sizeof(Real);  // complex pairs per 16-byte chunk
+  union Chunk {
+    V v;
+    Real r[2 * CPL];
+  };
+  const char* tb = reinterpret_cast<const char*>(cube_rows<Real>(d));
+  Chunk rows[8];
+#pragma unroll
+  for (int x = 0; x < 8; ++x) rows[x].v = *reinterpret_cast<const V*>(tb + (off + (uint32_t)x * slot_bytes));
+  Real acc[2 * CPL];
+#pragma unroll
+  for (int k = 0; k < 2 * CPL; ++k) acc[k] = 0;
+#pragma unroll
+  for (int x = 0; x < 8; ++x) {
+    const Real w = r.w[x];
+#pragma unroll
+    for (int k = 0; k < 2 * CPL; ++k) acc[k] = fma_r(w, rows[x].r[k], acc[k]);
+  }
+  // values: pair P >= SC holds values 2 (P - SC) and 2 (P - SC) + 1
+#pragma unroll
+  for (int h = 0; h < CPL; ++h) {
+    const int vi = 2 * ((int)q * CPL + h - SC);
+    if (vi >= 0 && vi < SV) ov[vi] = acc[2 * h];
+    if (vi >= 0 && vi + 1 < SV) ov[vi + 1] = acc[2 * h + 1];
+  }
+  if constexpr (GAMMA) {
+    C all[SC];
+    grp_gather<Real, G, CPL, SC>(acc, all);
+#pragma unroll
+    for (int h = 0; h < CPL; ++h) {
+      const uint32_t m = q * CPL + h;
+      if (m >= (uint32_t)SC) continue;
+      // output element m: component j of output atom m / 3, taken from the
+      // source atom a the GammaTable maps there
+      const uint32_t oa = m / 3u, j = m - 3u * oa;
+      const uint32_t a = fr->from[oa];
+      if (a == 255u) continue;  // no source atom: unit_onepass leaves the element unwritten too
+      C v[3];
+      v[0] = v[1] = v[2] = all[0];
+#pragma unroll
+      for (int s = 0; s < SC / 3; ++s) {
+        if ((uint32_t)s == a) {
+          v[0] = all[3 * s];
+          v[1] = all[3 * s + 1];
+          v[2] = all[3 * s + 2];
+        }
+      }
+      const Real* R = fr->R + 3 * j;
+      C t = Elem<C>::zero();
+#pragma unroll
+      for (int k = 0; k < 3; ++k) t = Elem<C>::add(t, Elem<C>::scale(v[k], R[k]));
+      oc[m] = cmul(phases[a], t);
+    }
+  } else {
+#pragma unroll
+    for (int h = 0; h < CPL; ++h)
+      if (q * CPL + h < (uint32_t)SC) oc[q * CPL + h] = C{acc[2 * h], acc[2 * h + 1]};
   }
 }
 
@@ -831,18 +1020,20 @@ __device__ inline void write_aux(const brille_aux_out& aux, uint64_t p, const Lo
 // compile-time row sizes of the specialised layouts (ignored by
 // LAYOUT_GENERIC).
 
-template <int L, int SC, int TB = TILE>
+template <int L, int SC, int TB = TILE, typename Real = double>
 struct TileShape {
   static constexpr int NPH = L == LAYOUT_GAMMA ? SC / 3 : (L == LAYOUT_GENERIC ? 4 : 1);  // phases per point
-  static constexpr int NQ = (L == LAYOUT_GENERIC || L == LAYOUT_WIDE) ? 3 * TB : 1;      // q kept in LDS
+  // per-point side data in s_q: q (general and wide layouts) or the GammaFrame
+  static constexpr int NQ = (L == LAYOUT_GENERIC || L == LAYOUT_WIDE) ? 3 * TB
+                            : (L == LAYOUT_GAMMA ? frame_words<Real>() * TB : 1);
   static constexpr int NW = L == LAYOUT_WIDE ? (TB / 64) * (int)kWideMax : 1;              // wide-row slices
 };
 
 // a located point's phase-2 record (plus phases, and q for the general layout)
 template <typename Real, int L, int SC>
-__device__ __forceinline__ void store_record(const DataConst& d, const Located& P, PointRec<Real>& r,
-                                             typename Cplx<Real>::T* s_ph, double* s_q, uint32_t slot,
-                                             bool ph_in_lds, uint32_t n_atoms) {
+__device__ __forceinline__ void store_record(const GridConst& g, const DataConst& d, const Located& P,
+                                             PointRec<Real>& r, typename Cplx<Real>::T* s_ph, double* s_q,
+                                             uint32_t slot, bool ph_in_lds, uint32_t n_atoms) {
   constexpr int NPH = TileShape<L, SC>::NPH;
 #pragma unroll
   for (int s = 0; s < 8; ++s) {
@@ -857,6 +1048,28 @@ __device__ __forceinline__ void store_record(const DataConst& d, const Located& 
   if constexpr (L == LAYOUT_GAMMA) {
 #pragma unroll
     for (int k = 0; k < NPH; ++k) s_ph[NPH * slot + k] = gamma_phase<Real>(d, P.q, P.invrot, k);
+    GammaFrame<Real>& f = reinterpret_cast<GammaFrame<Real>*>(s_q)[slot];
+    if (d.vec_lu == BRILLE_LU_ANGSTROM) {
+#pragma unroll
+      for (int k = 0; k < 9; ++k) f.R[k] = (Real)d.cart_rot[9 * (uint64_t)P.invrot + k];
+    } else {
+#pragma unroll
+      for (int k = 0; k < 9; ++k) f.R[k] = (Real)g.rot[9 * (uint64_t)P.invrot + k];
+    }
+    uint8_t to[4] = {255, 255, 255, 255}, from[4] = {255, 255, 255, 255};
+#pragma unroll
+    for (int a = 0; a < SC / 3; ++a) {
+      const uint32_t l = d.F0[a * d.gamma_ops + P.invrot];
+      to[a] = (uint8_t)l;
+#pragma unroll
+      for (int o = 0; o < SC / 3; ++o)
+        if ((uint32_t)o == l) from[o] = (uint8_t)a;
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      f.to[a] = to[a];
+      f.from[a] = from[a];
+    }
   } else if constexpr (L == LAYOUT_GENERIC) {
     s_q[3 * slot] = P.q[0]; s_q[3 * slot + 1] = P.q[1]; s_q[3 * slot + 2] = P.q[2];
     if (ph_in_lds)
@@ -869,12 +1082,14 @@ __device__ __forceinline__ void store_record(const DataConst& d, const Located& 
 // phase 2 over the npts records of a tile of TB points (TB threads): point pl
 // writes output row out_row[pl] (or row0 + pl when out_row is null); GRP
 // overrides the rows the one-pass sums keep in flight (0: their default)
+// table: the cube factor table is built; its points are the first n_coop
+// records, summed by unit_coop where the rows allow it
 template <typename Real, bool Fast, int L, int SC, int SV, int TB = TILE, int GRP = 0>
 __device__ __forceinline__ void tile_sums(const GridConst& g, const DataConst& d, const PointRec<Real>* s_rec,
                                           const typename Cplx<Real>::T* s_ph, const double* s_q,
                                           const uint32_t* out_row, uint64_t row0, uint32_t npts, bool ph_in_lds,
                                           void* vals_, void* vecs_, typename Cplx<Real>::T* s_wide = nullptr,
-                                          bool table = false) {
+                                          bool table = false, uint32_t n_coop = 0) {
   using C = typename Cplx<Real>::T;
   constexpr int NPH = TileShape<L, SC>::NPH;
   const uint32_t B = d.B;
@@ -902,7 +1117,37 @@ __device__ __forceinline__ void tile_sums(const GridConst& g, const DataConst& d
     }
     return;
   }
-  for (uint32_t u = threadIdx.x; u < units; u += TB) {
+  const GammaFrame<Real>* frames = reinterpret_cast<const GammaFrame<Real>*>(s_q);
+  uint32_t u0 = 0;
+  constexpr bool kCoop = (L == LAYOUT_GAMMA || L == LAYOUT_SCALAR) && coop_lanes<Real, SC, SV>() > 1;
+  if constexpr (kCoop) {
+    constexpr uint32_t G = coop_lanes<Real, SC, SV>();
+    constexpr uint32_t STEP = TB / G;  // units per block and iteration
+    if (table) {
+      u0 = n_coop * B;
+      const uint32_t q = threadIdx.x & (G - 1u);
+      // (point, mode) of unit u, advanced by STEP units per iteration without a division
+      uint32_t u = threadIdx.x / G;
+      uint32_t pl = u / B, b = u - pl * B;
+      const uint32_t sp = STEP / B, sb = STEP - sp * B;
+      const uint32_t rb = payload_stride<Real>(d) * (uint32_t)sizeof(Real);  // bytes per row
+      const uint32_t slot_bytes = B * rb;
+      for (; u < u0; u += STEP) {
+        const PointRec<Real>& r = s_rec[pl];
+        const uint64_t row = ((out_row ? (uint64_t)out_row[pl] : row0 + pl)) * B + b;
+        const uint32_t off = (r.cube * 8u * B + b) * rb + q * 16u;
+        unit_coop<Real, SC, SV, L == LAYOUT_GAMMA, (int)G>(d, r, off, slot_bytes, q, vals + row * d.sv,
+                                                           vecs + row * d.sc, s_ph + NPH * pl, frames + pl);
+        b += sb;
+        pl += sp;
+        if (b >= B) {
+          b -= B;
+          ++pl;
+        }
+      }
+    }
+  }
+  for (uint32_t u = u0 + threadIdx.x; u < units; u += TB) {
     const uint32_t pl = u / B, b = u - pl * B;
     const PointRec<Real>& r = s_rec[pl];
     const uint64_t row = ((out_row ? (uint64_t)out_row[pl] : row0 + pl)) * B + b;
@@ -914,10 +1159,11 @@ __device__ __forceinline__ void tile_sums(const GridConst& g, const DataConst& d
       for (uint32_t e = 0; e < d.sc; ++e) oc[e] = Elem<C>::zero();
       continue;
     }
+    // with the cooperative sums, every table point went there
     if constexpr (L == LAYOUT_GAMMA) {
-      unit_onepass<Real, SC, SV, true, GRP>(g, d, r, b, ov, oc, s_ph + NPH * pl, cart, table);
+      unit_onepass<Real, SC, SV, true, GRP>(d, r, b, ov, oc, s_ph + NPH * pl, frames + pl, table && !kCoop);
     } else if constexpr (L == LAYOUT_SCALAR) {
-      unit_onepass<Real, SC, SV, false, GRP>(g, d, r, b, ov, oc, nullptr, cart, table);
+      unit_onepass<Real, SC, SV, false, GRP>(d, r, b, ov, oc, nullptr, nullptr, table && !kCoop);
     } else {
       unit_generic<Real, !Fast>(g, d, r, s_q + 3 * pl, b, ov, oc, ph_in_lds ? s_ph + NPH * pl : nullptr);
     }
@@ -937,7 +1183,7 @@ __device__ __forceinline__ void report_point(ErrRec* err, uint64_t index, int st
 template <typename Real, bool Fast, int L, int SC, int SV>
 __global__ __launch_bounds__(TILE) void k_ir_interpolate(GridConst g, DataConst d, LaunchArgs a) {
   using C = typename Cplx<Real>::T;
-  using S = TileShape<L, SC>;
+  using S = TileShape<L, SC, TILE, Real>;
   __shared__ PointRec<Real> s_rec[TILE];
   __shared__ C s_ph[TILE * S::NPH];
   __shared__ double s_q[S::NQ];
@@ -962,7 +1208,7 @@ __global__ __launch_bounds__(TILE) void k_ir_interpolate(GridConst g, DataConst 
     if (live) {
       write_aux(a.aux, p, P);
       if (P.status != BRILLE_PT_OK) report_point(a.err, a.base + p, P.status);
-      else store_record<Real, L, SC>(d, P, r, s_ph, s_q, tid, ph_in_lds, n_atoms);
+      else store_record<Real, L, SC>(g, d, P, r, s_ph, s_q, tid, ph_in_lds, n_atoms);
     }
   }
   __syncthreads();
