@@ -175,7 +175,7 @@ __global__ __launch_bounds__(BIN_BLOCK) void k_scan_apply(uint32_t* a, uint64_t 
 
 __global__ __launch_bounds__(BIN_BLOCK) void k_bin_scatter(const FoldRec* rec, uint32_t n, const uint32_t* off,
                                                            uint32_t nblk, uint32_t nb, uint64_t n_nodes, SortRec* out,
-                                                           const uint32_t* hint) {
+                                                           uint32_t* out_node, const uint32_t* hint) {
   extern __shared__ uint32_t s[];  // base[nb + 1], count[nb + 1]
   uint32_t* base = s;
   uint32_t* cnt = s + nb + 1;
@@ -198,8 +198,50 @@ __global__ __launch_bounds__(BIN_BLOCK) void k_bin_scatter(const FoldRec* rec, u
       o.index = p;
       o.hint = (hint && (r.meta & kMetaPoly)) ? hint[p] : kNoTet;
       out[base[k] + j] = o;
+      out_node[base[k] + j] = r.node;
     }
   }
+}
+
+// Node order inside each bucket: one block per bucket counts its points per
+// trellis node in LDS and writes perm, the positions of the bucket's records
+// in node order, so that the points of one cube are summed together and its
+// rows are read from L1.  Within a node the order is whatever the LDS atomics
+// give (the points are independent).  Buckets spanning more than kPermNodes
+// nodes, and the failed-point bucket, keep their order.
+constexpr uint32_t kPermNodes = 4096;
+__global__ __launch_bounds__(BIN_BLOCK) void k_bucket_perm(const uint32_t* node, uint32_t m, const uint32_t* off,
+                                                           uint32_t nblk, uint32_t nb, uint64_t n_nodes,
+                                                           uint32_t* perm) {
+  __shared__ uint32_t cnt[kPermNodes];
+  const uint32_t b = blockIdx.x;
+  const uint32_t s0 = off[(uint64_t)b * nblk];
+  const uint32_t s1 = b + 1 <= nb ? off[(uint64_t)(b + 1) * nblk] : m;
+  const uint64_t lo = ((uint64_t)b * n_nodes + nb - 1) / nb;
+  const uint64_t hi = b < nb ? ((uint64_t)(b + 1) * n_nodes + nb - 1) / nb : lo;
+  const uint32_t nn = (uint32_t)(hi - lo);
+  if (b == nb || nn == 0 || nn > kPermNodes) {
+    for (uint32_t i = s0 + threadIdx.x; i < s1; i += BIN_BLOCK) perm[i] = i;
+    return;
+  }
+  for (uint32_t k = threadIdx.x; k < nn; k += BIN_BLOCK) cnt[k] = 0;
+  __syncthreads();
+  for (uint32_t i = s0 + threadIdx.x; i < s1; i += BIN_BLOCK) atomicAdd(&cnt[node[i] - (uint32_t)lo], 1u);
+  __syncthreads();
+  // exclusive scan of cnt[0, nn): each thread a contiguous run, then the runs
+  const uint32_t per = (nn + BIN_BLOCK - 1) / BIN_BLOCK;
+  const uint32_t k0 = threadIdx.x * per, k1 = k0 + per < nn ? k0 + per : nn;
+  uint32_t run = 0;
+  for (uint32_t k = k0; k < k1; ++k) run += cnt[k];
+  uint32_t total;
+  uint32_t acc = s0 + block_exclusive_scan(run, &total);
+  for (uint32_t k = k0; k < k1; ++k) {
+    const uint32_t c = cnt[k];
+    cnt[k] = acc;
+    acc += c;
+  }
+  __syncthreads();
+  for (uint32_t i = s0 + threadIdx.x; i < s1; i += BIN_BLOCK) perm[atomicAdd(&cnt[node[i] - (uint32_t)lo], 1u)] = i;
 }
 
 // The learned proposal for the listed points only (those in polyhedron
@@ -246,8 +288,8 @@ __global__ __launch_bounds__(256) void k_bin_propose(GridConst g, ProposalConst 
 
 template <typename Real, int L, int SC, int SV, bool Prop>
 __global__ __launch_bounds__(TILE) void k_bin_sums(GridConst g, DataConst d, ProposalConst pc, const SortRec* srt,
-                                                    uint32_t n, void* vals, void* vecs, brille_aux_out aux,
-                                                    ErrRec* err, uint64_t base) {
+                                                    const uint32_t* perm, uint32_t n, void* vals, void* vecs,
+                                                    brille_aux_out aux, ErrRec* err, uint64_t base) {
   using C = typename Cplx<Real>::T;
   using S = TileShape<L, SC, TILE, Real>;
   __shared__ PointRec<Real> s_rec[TILE];
@@ -274,7 +316,7 @@ __global__ __launch_bounds__(TILE) void k_bin_sums(GridConst g, DataConst d, Pro
   P.status = BRILLE_PT_OK;
   P.cube = kNoCube;
   if (live) {
-    sr = srt[p];
+    sr = srt[perm[p]];
     P.q[0] = sr.q[0]; P.q[1] = sr.q[1]; P.q[2] = sr.q[2];
     P.tau[0] = P.tau[1] = P.tau[2] = 0;
     P.node = sr.node;
@@ -302,7 +344,7 @@ __global__ __launch_bounds__(TILE) void k_bin_sums(GridConst g, DataConst d, Pro
   // Stable partition of the tile: points the cube factor table serves first,
   // so that few waves of the sums mix the two kinds of point.
   const bool table = (L == LAYOUT_GAMMA || L == LAYOUT_SCALAR) && cube_rows<Real>(d) != nullptr;
-  const bool tab = table && live && P.status == BRILLE_PT_OK && P.cube != kNoCube;
+  const bool tab = table && live && P.status == BRILLE_PT_OK && table_row(d, P) != kNoCube;
   const uint32_t lane = tid & 63u, wave = tid >> 6;
   const unsigned long long bal = __ballot(tab);
   const uint32_t rank = (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
@@ -338,23 +380,34 @@ __global__ __launch_bounds__(TILE) void k_bin_sums(GridConst g, DataConst d, Pro
                                               table, n_tab);
 }
 
-// The cube row table: thread t = ((cube c, slot x), mode b) phases the row of
-// c's slot-x vertex against the row of its slot-0 vertex with the sums' own
-// functions, so a point summed from the table gets the same bits.
+// The row table: thread t = (block row, mode b) phases the row of a cube's or
+// tetrahedron's slot-x vertex against the row of its slot-0 vertex with the
+// sums' own functions, so a point summed from the table gets the same bits.
 template <typename Real, int SC>
-__global__ __launch_bounds__(256) void k_cube_rows(GridConst g, DataConst d, uint64_t n_units, Real* out) {
+__global__ __launch_bounds__(256) void k_table_rows(GridConst g, DataConst d, uint64_t n_cube_units,
+                                                    uint64_t n_units, Real* out) {
   const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n_units) return;
-  const uint64_t cx = t / d.B;
-  const uint32_t b = (uint32_t)(t - cx * d.B);
-  const uint64_t c = cx >> 3;
-  const int x = (int)(cx & 7u);
-  const uint32_t* v = g.cube_v + 8 * c;
+  uint32_t pv, sv;
+  uint32_t b;
+  if (t < n_cube_units) {
+    const uint64_t cx = t / d.B;
+    b = (uint32_t)(t - cx * d.B);
+    const uint32_t* v = g.cube_v + 8 * (cx >> 3);
+    pv = v[7];
+    sv = v[7 - (int)(cx & 7u)];
+  } else {
+    const uint64_t tx = (t - n_cube_units) / d.B;
+    b = (uint32_t)(t - n_cube_units - tx * d.B);
+    const uint32_t* v = g.tet_v + 4 * (tx >> 2);
+    pv = v[0];
+    sv = v[tx & 3u];
+  }
   const Real* base = payload_rows<Real>(d);
   const uint64_t stride = payload_stride<Real>(d);
   Real pr[2 * SC];
-  load_row<Real, 2 * SC>(pr, base + ((uint64_t)v[7] * d.B + b) * stride);
-  const Real* src = base + ((uint64_t)v[7 - x] * d.B + b) * stride;
+  load_row<Real, 2 * SC>(pr, base + ((uint64_t)pv * d.B + b) * stride);
+  const Real* src = base + ((uint64_t)sv * d.B + b) * stride;
   Real row[2 * SC];
   load_row<Real, 2 * SC>(row, src);
   const typename Cplx<Real>::T e = slot_antiphase<Real, SC>(pr, row);
@@ -363,8 +416,8 @@ __global__ __launch_bounds__(256) void k_cube_rows(GridConst g, DataConst d, uin
   for (uint32_t k = 2 * SC; k < stride; ++k) dst[k] = src[k];  // values and zero padding
 }
 
-using SumsKernel = void (*)(GridConst, DataConst, ProposalConst, const SortRec*, uint32_t, void*, void*,
-                           brille_aux_out, ErrRec*, uint64_t);
+using SumsKernel = void (*)(GridConst, DataConst, ProposalConst, const SortRec*, const uint32_t*, uint32_t, void*,
+                           void*, brille_aux_out, ErrRec*, uint64_t);
 
 template <typename Real>
 static SumsKernel pick_sums(const DataConst& d, bool prop) {
@@ -413,7 +466,8 @@ static uint64_t payload_stride_host(const DataConst& d) {
 template <typename Real>
 static int build_cube_rows(brille_grid* G, hipStream_t s, Real** out) {
   const DataConst& d = G->d;
-  const uint64_t units = G->n_cubes * 8 * d.B;
+  const uint64_t cube_units = G->n_cubes * 8 * d.B;
+  const uint64_t units = cube_units + G->n_tets * 4 * d.B;
   void* p = nullptr;
   if (hipMalloc(&p, units * payload_stride_host<Real>(d) * sizeof(Real)) != hipSuccess)
     return fail(BRILLE_ERR_DEVICE, "hipMalloc failed for the cube row table");
@@ -421,13 +475,13 @@ static int build_cube_rows(brille_grid* G, hipStream_t s, Real** out) {
   Real* t = static_cast<Real*>(p);
   const dim3 grid((unsigned)((units + 255) / 256));
   switch (onepass_sc(d)) {
-    case 1: hipLaunchKernelGGL((k_cube_rows<Real, 1>), grid, dim3(256), 0, s, G->g, d, units, t); break;
-    case 2: hipLaunchKernelGGL((k_cube_rows<Real, 2>), grid, dim3(256), 0, s, G->g, d, units, t); break;
-    case 3: hipLaunchKernelGGL((k_cube_rows<Real, 3>), grid, dim3(256), 0, s, G->g, d, units, t); break;
-    case 4: hipLaunchKernelGGL((k_cube_rows<Real, 4>), grid, dim3(256), 0, s, G->g, d, units, t); break;
-    case 6: hipLaunchKernelGGL((k_cube_rows<Real, 6>), grid, dim3(256), 0, s, G->g, d, units, t); break;
-    case 9: hipLaunchKernelGGL((k_cube_rows<Real, 9>), grid, dim3(256), 0, s, G->g, d, units, t); break;
-    case 12: hipLaunchKernelGGL((k_cube_rows<Real, 12>), grid, dim3(256), 0, s, G->g, d, units, t); break;
+    case 1: hipLaunchKernelGGL((k_table_rows<Real, 1>), grid, dim3(256), 0, s, G->g, d, cube_units, units, t); break;
+    case 2: hipLaunchKernelGGL((k_table_rows<Real, 2>), grid, dim3(256), 0, s, G->g, d, cube_units, units, t); break;
+    case 3: hipLaunchKernelGGL((k_table_rows<Real, 3>), grid, dim3(256), 0, s, G->g, d, cube_units, units, t); break;
+    case 4: hipLaunchKernelGGL((k_table_rows<Real, 4>), grid, dim3(256), 0, s, G->g, d, cube_units, units, t); break;
+    case 6: hipLaunchKernelGGL((k_table_rows<Real, 6>), grid, dim3(256), 0, s, G->g, d, cube_units, units, t); break;
+    case 9: hipLaunchKernelGGL((k_table_rows<Real, 9>), grid, dim3(256), 0, s, G->g, d, cube_units, units, t); break;
+    case 12: hipLaunchKernelGGL((k_table_rows<Real, 12>), grid, dim3(256), 0, s, G->g, d, cube_units, units, t); break;
     default: return fail(BRILLE_ERR_UNSUPPORTED, "no cube row kernel for this payload");
   }
   if (hipGetLastError() != hipSuccess) return fail(BRILLE_ERR_DEVICE, "cube row table launch failed");
@@ -442,9 +496,10 @@ static int build_cube_rows(brille_grid* G, hipStream_t s, Real** out) {
 constexpr uint64_t kCubeRowsMax = (4ull << 30) - 1;  // the sums address the table with 32-bit offsets
 int ensure_cube_rows(brille_grid* G, int precision, hipStream_t s) {
   DataConst& d = G->d;
-  if (G->n_cubes == 0 || d.B == 0 || onepass_sc(d) == 0) return BRILLE_OK;
+  if (G->n_cubes + G->n_tets == 0 || d.B == 0 || onepass_sc(d) == 0) return BRILLE_OK;
   const uint64_t row_bytes = precision == BRILLE_FP32 ? d.stride32 * 4ull : d.stride64 * 8ull;
-  if (G->n_cubes * 8 * d.B * row_bytes > kCubeRowsMax) return BRILLE_OK;
+  if ((G->n_cubes * 8 + G->n_tets * 4) * d.B * row_bytes > kCubeRowsMax) return BRILLE_OK;
+  d.tab_tet_row = (uint32_t)(G->n_cubes * 8 * d.B);
   if (precision == BRILLE_FP32) {
     if (d.cube_rows32) return BRILLE_OK;
     float* t = nullptr;
@@ -521,7 +576,8 @@ int launch_binned(brille_grid* G, const LaunchArgs& a, int precision, hipStream_
   auto al = [](uint64_t b) { return (b + 255) & ~uint64_t(255); };
   const uint64_t o_rec = 0, o_srt = al(chunk * sizeof(FoldRec)), o_hist = o_srt + al(chunk * sizeof(SortRec)),
                  o_part = o_hist + al(m_max * 4), o_list = o_part + al(np_max * 4), o_hint = o_list + al(chunk * 4),
-                 o_cnt = o_hint + al(chunk * 4), need = o_cnt + 256;
+                 o_perm = o_hint + al(chunk * 4), o_node = o_perm + al(chunk * 4), o_cnt = o_node + al(chunk * 4),
+                 need = o_cnt + 256;
   if (np_max > 256ull * 1024) return fail(BRILLE_ERR_INVALID_ARGUMENT, "binned pass too large");
   if (G->bin_ws_bytes < need) {
     free_binned(G);
@@ -537,6 +593,8 @@ int launch_binned(brille_grid* G, const LaunchArgs& a, int precision, hipStream_
   uint32_t* plist = prop ? reinterpret_cast<uint32_t*>(ws + o_list) : nullptr;
   uint32_t* phint = prop ? reinterpret_cast<uint32_t*>(ws + o_hint) : nullptr;
   uint32_t* pcount = reinterpret_cast<uint32_t*>(ws + o_cnt);
+  uint32_t* perm = reinterpret_cast<uint32_t*>(ws + o_perm);
+  uint32_t* snode = reinterpret_cast<uint32_t*>(ws + o_node);
   const SumsKernel sums = precision == BRILLE_FP32 ? pick_sums<float>(G->d, prop) : pick_sums<double>(G->d, prop);
   const size_t rs = precision == BRILLE_FP32 ? 4 : 8;
   const size_t vb = (size_t)G->d.B * G->d.sv * rs, cb = (size_t)G->d.B * G->d.sc * 2 * rs;
@@ -559,9 +617,11 @@ int launch_binned(brille_grid* G, const LaunchArgs& a, int precision, hipStream_
       hipLaunchKernelGGL(k_bin_propose, dim3(G->n_cu ? 4 * G->n_cu : 1024), dim3(256), 0, s, G->g, a.prop, rec, plist,
                          pcount, phint);
     hipLaunchKernelGGL(k_bin_scatter, dim3(nblk), dim3(BIN_BLOCK), 2 * lds1, s, rec, m, hist, nblk, nb,
-                       G->g.n_nodes, srt, phint);
+                       G->g.n_nodes, srt, snode, phint);
+    hipLaunchKernelGGL(k_bucket_perm, dim3(nb + 1), dim3(BIN_BLOCK), 0, s, snode, m, hist, nblk, nb, G->g.n_nodes,
+                       perm);
     if (evp) (void)hipEventRecord(evp[2], s);
-    hipLaunchKernelGGL(sums, dim3((m + TILE - 1) / TILE), dim3(TILE), 0, s, G->g, G->d, a.prop, srt, m,
+    hipLaunchKernelGGL(sums, dim3((m + TILE - 1) / TILE), dim3(TILE), 0, s, G->g, G->d, a.prop, srt, perm, m,
                        static_cast<char*>(a.vals) + c0 * vb, static_cast<char*>(a.vecs) + c0 * cb,
                        offset_aux(a.aux, c0), a.err, a.base + c0);
     if (evp) {

@@ -88,13 +88,15 @@ struct DataConst {
   const double* rows64;
   const float* rows32;
   uint32_t stride64, stride32;  // in reals
-  // Cube row table (null until built; read by the bucketed sums only): for
-  // cube c, slot x and mode b, at ((c * 8 + x) * B + b) * stride, the row of
-  // the cube's slot-x vertex with every vector element multiplied by its
-  // antiphase against the slot-0 vertex (the pivot of every cube point whose
-  // slot-0 weight is non-zero), values as they are; same stride as the rows.
+  // Row table (null until built; read by the bucketed sums only): for cube c,
+  // slot x and mode b, row (c * 8 + x) * B + b, and for tetrahedron t row
+  // tab_tet_row + (t * 4 + x) * B + b: the row of the slot-x vertex with every
+  // vector element multiplied by its antiphase against the slot-0 vertex (the
+  // pivot of every point whose slot-0 weight is non-zero), values as they
+  // are; same stride as the payload rows.
   const float* cube_rows32;
   const double* cube_rows64;
+  uint32_t tab_tet_row;
   // GammaTable
   uint32_t n_atoms, gamma_ops;
   const uint32_t* F0;

@@ -66,7 +66,7 @@ struct PointRec {
   uint32_t idx[8];
   Real w[8];
   uint32_t pivot;
-  uint32_t cube;  // cube whose slot-0 antiphase factors apply, or kNoCube
+  uint32_t trow;  // row-table row of slot 0 for mode 0 (kNoCube: the point has no rows there)
   uint16_t rot, invrot;
   uint8_t nslot, pad0, pad1, pad2;
 };
@@ -555,10 +555,10 @@ __device__ __forceinline__ const float* cube_rows<float>(const DataConst& d) { r
 template <>
 __device__ __forceinline__ const double* cube_rows<double>(const DataConst& d) { return d.cube_rows64; }
 
-// antiphased row of slot x of cube c for mode b in the cube row table
+// antiphased row of slot x for mode b of the block starting at table row trow
 template <typename Real>
-__device__ __forceinline__ const Real* cube_row_at(const DataConst& d, uint32_t c, int x, uint32_t b) {
-  return cube_rows<Real>(d) + (((uint64_t)c * 8 + (uint64_t)x) * d.B + b) * payload_stride<Real>(d);
+__device__ __forceinline__ const Real* table_row_at(const DataConst& d, uint32_t trow, int x, uint32_t b) {
+  return cube_rows<Real>(d) + ((uint64_t)trow + (uint64_t)x * d.B + b) * payload_stride<Real>(d);
 }
 
 // The antiphase of slot x's row against the pivot row p: Hermitian product and
@@ -588,7 +588,7 @@ __device__ __forceinline__ void unit_onepass(const DataConst& d, const PointRec<
   const Real* base = payload_rows<Real>(d);
   const uint64_t stride = payload_stride<Real>(d);
   // only cube points with the pivot in slot 0 have rows in the table
-  const bool tab = table && r.cube != kNoCube;
+  const bool tab = table && r.trow != kNoCube;
   Real pr[2 * SC];
   if (!tab) load_row<Real, 2 * SC>(pr, base + ((uint64_t)r.pivot * d.B + b) * stride);
   Real acc[2 * SC];
@@ -607,7 +607,15 @@ __device__ __forceinline__ void unit_onepass(const DataConst& d, const PointRec<
     Real row[GR][NR];
     if (tab) {
 #pragma unroll
-      for (int x = 0; x < GR; ++x) load_row<Real, NR>(row[x], cube_row_at<Real>(d, r.cube, x0 + x, b));
+      for (int x = 0; x < GR; ++x) {
+        // a tetrahedron's block has four slots; the others weigh 0
+        if (x0 + x < r.nslot) {
+          load_row<Real, NR>(row[x], table_row_at<Real>(d, r.trow, x0 + x, b));
+        } else {
+#pragma unroll
+          for (int k = 0; k < NR; ++k) row[x][k] = 0;
+        }
+      }
     } else {
 #pragma unroll
       for (int x = 0; x < GR; ++x) load_row<Real, NR>(row[x], base + ((uint64_t)r.idx[x0 + x] * d.B + b) * stride);
@@ -729,7 +737,15 @@ __device__ __forceinline__ void unit_coop(const DataConst& d, const PointRec<Rea
   const char* tb = reinterpret_cast<const char*>(cube_rows<Real>(d));
   Chunk rows[8];
 #pragma unroll
-  for (int x = 0; x < 8; ++x) rows[x].v = *reinterpret_cast<const V*>(tb + (off + (uint32_t)x * slot_bytes));
+  for (int x = 0; x < 8; ++x) {
+    // a tetrahedron's block has four slots; the others weigh 0
+    if (x < 4 || r.nslot == 8) {
+      rows[x].v = *reinterpret_cast<const V*>(tb + (off + (uint32_t)x * slot_bytes));
+    } else {
+#pragma unroll
+      for (int k = 0; k < 2 * CPL; ++k) rows[x].r[k] = 0;
+    }
+  }
   Real acc[2 * CPL];
 #pragma unroll
   for (int k = 0; k < 2 * CPL; ++k) acc[k] = 0;
@@ -1029,6 +1045,15 @@ struct TileShape {
   static constexpr int NW = L == LAYOUT_WIDE ? (TB / 64) * (int)kWideMax : 1;              // wide-row slices
 };
 
+// The row-table block of a point: cube c's eight slots at rows c * 8 * B, the
+// four slots of tetrahedron t at tab_tet_row + t * 4 * B; only points whose
+// antiphase pivot is slot 0 have one.
+__device__ __forceinline__ uint32_t table_row(const DataConst& d, const Located& P) {
+  if (P.cube != kNoCube) return P.cube * 8u * d.B;
+  if (P.tet != kNoTet && P.nslot == 4 && P.w[0] != 0.0) return d.tab_tet_row + P.tet * 4u * d.B;
+  return kNoCube;
+}
+
 // a located point's phase-2 record (plus phases, and q for the general layout)
 template <typename Real, int L, int SC>
 __device__ __forceinline__ void store_record(const GridConst& g, const DataConst& d, const Located& P,
@@ -1041,7 +1066,7 @@ __device__ __forceinline__ void store_record(const GridConst& g, const DataConst
     r.w[s] = (Real)P.w[s];
   }
   r.pivot = P.pivot;
-  r.cube = P.cube;
+  r.trow = table_row(d, P);
   r.rot = (uint16_t)P.rot;
   r.invrot = (uint16_t)P.invrot;
   r.nslot = (uint8_t)P.nslot;
@@ -1135,7 +1160,7 @@ __device__ __forceinline__ void tile_sums(const GridConst& g, const DataConst& d
       for (; u < u0; u += STEP) {
         const PointRec<Real>& r = s_rec[pl];
         const uint64_t row = ((out_row ? (uint64_t)out_row[pl] : row0 + pl)) * B + b;
-        const uint32_t off = (r.cube * 8u * B + b) * rb + q * 16u;
+        const uint32_t off = (r.trow + b) * rb + q * 16u;
         unit_coop<Real, SC, SV, L == LAYOUT_GAMMA, (int)G>(d, r, off, slot_bytes, q, vals + row * d.sv,
                                                            vecs + row * d.sc, s_ph + NPH * pl, frames + pl);
         b += sb;
