@@ -721,21 +721,21 @@ __device__ __forceinline__ void grp_gather(const Real* acc, typename Cplx<Real>:
   }
 }
 
-// off: byte offset of chunk q of the unit's slot-0 row in the table (below
-// 4 GiB), slot_bytes: bytes from one slot's rows to the next (8 B rows)
-template <typename Real, int SC, int SV, bool GAMMA, int G>
-__device__ __forceinline__ void unit_coop(const DataConst& d, const PointRec<Real>& r, uint32_t off,
-                                          uint32_t slot_bytes, uint32_t q, Real* ov, typename Cplx<Real>::T* oc,
-                                          const typename Cplx<Real>::T* phases, const GammaFrame<Real>* fr) {
-  using C = typename Cplx<Real>::T;
+// one lane's 16-byte chunk of a row
+template <typename Real>
+union Chunk {
+  typename Cplx<Real>::V16 v;
+  Real r[16 / sizeof(Real)];
+};
+
+// chunk q of the eight rows of a unit: off is the byte offset of chunk q of
+// the slot-0 row in the table (below 4 GiB), slot_bytes the bytes from one
+// slot's rows to the next (B rows)
+template <typename Real>
+__device__ __forceinline__ void coop_load(const DataConst& d, const PointRec<Real>& r, uint32_t off,
+                                          uint32_t slot_bytes, Chunk<Real>* rows) {
   using V = typename Cplx<Real>::V16;
-  constexpr int CPL = 8 / (int)sizeof(Real);  // complex pairs per 16-byte chunk
-  union Chunk {
-    V v;
-    Real r[2 * CPL];
-  };
   const char* tb = reinterpret_cast<const char*>(cube_rows<Real>(d));
-  Chunk rows[8];
 #pragma unroll
   for (int x = 0; x < 8; ++x) {
     // a tetrahedron's block has four slots; the others weigh 0
@@ -743,9 +743,18 @@ __device__ __forceinline__ void unit_coop(const DataConst& d, const PointRec<Rea
       rows[x].v = *reinterpret_cast<const V*>(tb + (off + (uint32_t)x * slot_bytes));
     } else {
 #pragma unroll
-      for (int k = 0; k < 2 * CPL; ++k) rows[x].r[k] = 0;
+      for (int k = 0; k < (int)(16 / sizeof(Real)); ++k) rows[x].r[k] = 0;
     }
   }
+}
+
+// the weighted sum of a unit's chunks and the unit's outputs (see coop_load)
+template <typename Real, int SC, int SV, bool GAMMA, int G>
+__device__ __forceinline__ void unit_coop(const PointRec<Real>& r, const Chunk<Real>* rows, uint32_t q, Real* ov,
+                                          typename Cplx<Real>::T* oc, const typename Cplx<Real>::T* phases,
+                                          const GammaFrame<Real>* fr) {
+  using C = typename Cplx<Real>::T;
+  constexpr int CPL = 8 / (int)sizeof(Real);  // complex pairs per 16-byte chunk
   Real acc[2 * CPL];
 #pragma unroll
   for (int k = 0; k < 2 * CPL; ++k) acc[k] = 0;
@@ -1147,27 +1156,50 @@ __device__ __forceinline__ void tile_sums(const GridConst& g, const DataConst& d
   constexpr bool kCoop = (L == LAYOUT_GAMMA || L == LAYOUT_SCALAR) && coop_lanes<Real, SC, SV>() > 1;
   if constexpr (kCoop) {
     constexpr uint32_t G = coop_lanes<Real, SC, SV>();
-    constexpr uint32_t STEP = TB / G;  // units per block and iteration
     if (table) {
       u0 = n_coop * B;
       const uint32_t q = threadIdx.x & (G - 1u);
-      // (point, mode) of unit u, advanced by STEP units per iteration without a division
-      uint32_t u = threadIdx.x / G;
-      uint32_t pl = u / B, b = u - pl * B;
-      const uint32_t sp = STEP / B, sb = STEP - sp * B;
       const uint32_t rb = payload_stride<Real>(d) * (uint32_t)sizeof(Real);  // bytes per row
       const uint32_t slot_bytes = B * rb;
-      for (; u < u0; u += STEP) {
-        const PointRec<Real>& r = s_rec[pl];
-        const uint64_t row = ((out_row ? (uint64_t)out_row[pl] : row0 + pl)) * B + b;
-        const uint32_t off = (r.trow + b) * rb + q * 16u;
-        unit_coop<Real, SC, SV, L == LAYOUT_GAMMA, (int)G>(d, r, off, slot_bytes, q, vals + row * d.sv,
-                                                           vecs + row * d.sc, s_ph + NPH * pl, frames + pl);
-        b += sb;
-        pl += sp;
-        if (b >= B) {
-          b -= B;
-          ++pl;
+      constexpr uint32_t NG = TB / G;  // lane groups per block
+      Chunk<Real> rows[8];
+      if (B <= NG) {
+        // Group grp keeps mode b = grp % B and walks the points grp / B, + P,
+        // + 2P, ...: with the points in node order it meets the same cube or
+        // tetrahedron again and again, and reloads the rows only when the
+        // block changes.
+        const uint32_t P = NG / B;
+        const uint32_t grp = threadIdx.x / G;
+        const uint32_t b = grp % B;
+        uint32_t cached = kNoCube;
+        for (uint32_t pl = grp / B; pl < n_coop && grp < P * B; pl += P) {
+          const PointRec<Real>& r = s_rec[pl];
+          if (r.trow != cached) {
+            coop_load<Real>(d, r, (r.trow + b) * rb + q * 16u, slot_bytes, rows);
+            cached = r.trow;
+          }
+          const uint64_t row = ((out_row ? (uint64_t)out_row[pl] : row0 + pl)) * B + b;
+          unit_coop<Real, SC, SV, L == LAYOUT_GAMMA, (int)G>(r, rows, q, vals + row * d.sv, vecs + row * d.sc,
+                                                             s_ph + NPH * pl, frames + pl);
+        }
+      } else {
+        constexpr uint32_t STEP = NG;  // units per block and iteration
+        // (point, mode) of unit u, advanced by STEP units per iteration without a division
+        uint32_t u = threadIdx.x / G;
+        uint32_t pl = u / B, b = u - pl * B;
+        const uint32_t sp = STEP / B, sb = STEP - sp * B;
+        for (; u < u0; u += STEP) {
+          const PointRec<Real>& r = s_rec[pl];
+          const uint64_t row = ((out_row ? (uint64_t)out_row[pl] : row0 + pl)) * B + b;
+          coop_load<Real>(d, r, (r.trow + b) * rb + q * 16u, slot_bytes, rows);
+          unit_coop<Real, SC, SV, L == LAYOUT_GAMMA, (int)G>(r, rows, q, vals + row * d.sv, vecs + row * d.sc,
+                                                             s_ph + NPH * pl, frames + pl);
+          b += sb;
+          pl += sp;
+          if (b >= B) {
+            b -= B;
+            ++pl;
+          }
         }
       }
     }
