@@ -53,9 +53,11 @@ __device__ __forceinline__ uint32_t bucket_of(uint32_t node, uint32_t meta, uint
 // one atomic)
 template <bool Fast>
 __global__ __launch_bounds__(BIN_BLOCK) void k_bin_fold(GridConst g, const double* Q, uint32_t n, int fold,
-                                                        FoldRec* rec, uint32_t* hist, uint32_t nblk, uint32_t nb,
-                                                        ErrRec* err, uint64_t base, uint32_t* poly_list,
-                                                        uint32_t* poly_count) {
+                                                        FoldRec* __restrict__ rec, uint32_t* __restrict__ hist,
+                                                        uint32_t nblk, uint32_t nb,
+                                                        ErrRec* __restrict__ err, uint64_t base,
+                                                        uint32_t* __restrict__ poly_list,
+                                                        uint32_t* __restrict__ poly_count) {
   extern __shared__ uint32_t h[];  // nb + 1 counters
   __shared__ uint32_t s_poly[BIN_SPAN];
   __shared__ uint32_t s_npoly, s_at;
@@ -288,8 +290,9 @@ __global__ __launch_bounds__(256) void k_bin_propose(GridConst g, ProposalConst 
 
 template <typename Real, int L, int SC, int SV, bool Prop>
 __global__ __launch_bounds__(TILE) void k_bin_sums(GridConst g, DataConst d, ProposalConst pc, const SortRec* srt,
-                                                    const uint32_t* perm, uint32_t n, void* vals, void* vecs,
-                                                    brille_aux_out aux, ErrRec* err, uint64_t base) {
+                                                    const uint32_t* perm, uint32_t n, void* __restrict__ vals,
+                                                    void* __restrict__ vecs, brille_aux_out aux,
+                                                    ErrRec* __restrict__ err, uint64_t base) {
   using C = typename Cplx<Real>::T;
   using S = TileShape<L, SC, TILE, Real>;
   __shared__ PointRec<Real> s_rec[TILE];

@@ -771,18 +771,22 @@ __device__ __forceinline__ void unit_coop(const PointRec<Real>& r, const Chunk<R
     if (vi >= 0 && vi < SV) ov[vi] = acc[2 * h];
     if (vi >= 0 && vi + 1 < SV) ov[vi + 1] = acc[2 * h + 1];
   }
+  C out[CPL];
+  bool put[CPL];
   if constexpr (GAMMA) {
     C all[SC];
     grp_gather<Real, G, CPL, SC>(acc, all);
 #pragma unroll
     for (int h = 0; h < CPL; ++h) {
       const uint32_t m = q * CPL + h;
-      if (m >= (uint32_t)SC) continue;
+      out[h] = all[0];
       // output element m: component j of output atom m / 3, taken from the
-      // source atom a the GammaTable maps there
+      // source atom a the GammaTable maps there (none: unit_onepass leaves the
+      // element unwritten too)
       const uint32_t oa = m / 3u, j = m - 3u * oa;
-      const uint32_t a = fr->from[oa];
-      if (a == 255u) continue;  // no source atom: unit_onepass leaves the element unwritten too
+      const uint32_t a = m < (uint32_t)SC ? fr->from[oa < 4u ? oa : 0u] : 255u;
+      put[h] = a != 255u;
+      if (!put[h]) continue;
       C v[3];
       v[0] = v[1] = v[2] = all[0];
 #pragma unroll
@@ -797,13 +801,27 @@ __device__ __forceinline__ void unit_coop(const PointRec<Real>& r, const Chunk<R
       C t = Elem<C>::zero();
 #pragma unroll
       for (int k = 0; k < 3; ++k) t = Elem<C>::add(t, Elem<C>::scale(v[k], R[k]));
-      oc[m] = cmul(phases[a], t);
+      out[h] = cmul(phases[a], t);
     }
   } else {
 #pragma unroll
-    for (int h = 0; h < CPL; ++h)
-      if (q * CPL + h < (uint32_t)SC) oc[q * CPL + h] = C{acc[2 * h], acc[2 * h + 1]};
+    for (int h = 0; h < CPL; ++h) {
+      out[h] = C{acc[2 * h], acc[2 * h + 1]};
+      put[h] = q * CPL + h < (uint32_t)SC;
+    }
   }
+  // the lane's elements in one 16-byte store where they are whole and aligned
+  C* dst = oc + q * CPL;
+  if constexpr (CPL == 2) {
+    if (put[0] && put[1] && (reinterpret_cast<uintptr_t>(dst) & 15u) == 0) {
+      *reinterpret_cast<typename Cplx<Real>::V16*>(dst) =
+          typename Cplx<Real>::V16{out[0].x, out[0].y, out[1].x, out[1].y};
+      return;
+    }
+  }
+#pragma unroll
+  for (int h = 0; h < CPL; ++h)
+    if (put[h]) dst[h] = out[h];
 }
 
 // Payload layouts with a specialised phase 2 (the general one handles all).
