@@ -105,8 +105,9 @@ __global__ __launch_bounds__(BIN_BLOCK) void k_bin_fold(GridConst g, const doubl
 
 // exclusive scan of 256 values held one per thread; returns the thread's
 // exclusive prefix, *total gets the sum
+template <int NT = BIN_BLOCK>
 __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* total) {
-  __shared__ uint32_t wsum[BIN_BLOCK / 64];
+  __shared__ uint32_t wsum[NT / 64];
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
   uint32_t x = v;
 #pragma unroll
@@ -118,7 +119,7 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* t
   __syncthreads();
   uint32_t before = 0, all = 0;
 #pragma unroll
-  for (int w = 0; w < BIN_BLOCK / 64; ++w) {
+  for (int w = 0; w < NT / 64; ++w) {
     if ((uint32_t)w < wave) before += wsum[w];
     all += wsum[w];
   }
@@ -187,10 +188,19 @@ __global__ __launch_bounds__(BIN_BLOCK) void k_bin_scatter(const FoldRec* rec, u
   }
   __syncthreads();
   const uint32_t b0 = blockIdx.x * BIN_SPAN;
-  for (int i = 0; i < BIN_PPT; ++i) {
-    const uint32_t p = b0 + i * BIN_BLOCK + threadIdx.x;
+  constexpr int GRP = 8;  // records requested before any is placed
+  for (int i0 = 0; i0 < BIN_PPT; i0 += GRP) {
+    FoldRec rr[GRP];
+#pragma unroll
+    for (int i = 0; i < GRP; ++i) {
+      const uint32_t p = b0 + (i0 + i) * BIN_BLOCK + threadIdx.x;
+      if (p < n) rr[i] = rec[p];
+    }
+#pragma unroll
+    for (int i = 0; i < GRP; ++i) {
+    const uint32_t p = b0 + (i0 + i) * BIN_BLOCK + threadIdx.x;
     if (p < n) {
-      const FoldRec r = rec[p];
+      const FoldRec& r = rr[i];
       const uint32_t k = bucket_of(r.node, r.meta, nb, n_nodes);
       const uint32_t j = atomicAdd(&cnt[k], 1u);
       SortRec o;
@@ -202,6 +212,7 @@ __global__ __launch_bounds__(BIN_BLOCK) void k_bin_scatter(const FoldRec* rec, u
       out[base[k] + j] = o;
       out_node[base[k] + j] = r.node;
     }
+    }
   }
 }
 
@@ -212,7 +223,8 @@ __global__ __launch_bounds__(BIN_BLOCK) void k_bin_scatter(const FoldRec* rec, u
 // give (the points are independent).  Buckets spanning more than kPermNodes
 // nodes, and the failed-point bucket, keep their order.
 constexpr uint32_t kPermNodes = 4096;
-__global__ __launch_bounds__(BIN_BLOCK) void k_bucket_perm(const uint32_t* node, uint32_t m, const uint32_t* off,
+constexpr int PERM_BLOCK = 1024;
+__global__ __launch_bounds__(PERM_BLOCK) void k_bucket_perm(const uint32_t* node, uint32_t m, const uint32_t* off,
                                                            uint32_t nblk, uint32_t nb, uint64_t n_nodes,
                                                            uint32_t* perm) {
   __shared__ uint32_t cnt[kPermNodes];
@@ -223,27 +235,27 @@ __global__ __launch_bounds__(BIN_BLOCK) void k_bucket_perm(const uint32_t* node,
   const uint64_t hi = b < nb ? ((uint64_t)(b + 1) * n_nodes + nb - 1) / nb : lo;
   const uint32_t nn = (uint32_t)(hi - lo);
   if (b == nb || nn == 0 || nn > kPermNodes) {
-    for (uint32_t i = s0 + threadIdx.x; i < s1; i += BIN_BLOCK) perm[i] = i;
+    for (uint32_t i = s0 + threadIdx.x; i < s1; i += PERM_BLOCK) perm[i] = i;
     return;
   }
-  for (uint32_t k = threadIdx.x; k < nn; k += BIN_BLOCK) cnt[k] = 0;
+  for (uint32_t k = threadIdx.x; k < nn; k += PERM_BLOCK) cnt[k] = 0;
   __syncthreads();
-  for (uint32_t i = s0 + threadIdx.x; i < s1; i += BIN_BLOCK) atomicAdd(&cnt[node[i] - (uint32_t)lo], 1u);
+  for (uint32_t i = s0 + threadIdx.x; i < s1; i += PERM_BLOCK) atomicAdd(&cnt[node[i] - (uint32_t)lo], 1u);
   __syncthreads();
   // exclusive scan of cnt[0, nn): each thread a contiguous run, then the runs
-  const uint32_t per = (nn + BIN_BLOCK - 1) / BIN_BLOCK;
+  const uint32_t per = (nn + PERM_BLOCK - 1) / PERM_BLOCK;
   const uint32_t k0 = threadIdx.x * per, k1 = k0 + per < nn ? k0 + per : nn;
   uint32_t run = 0;
   for (uint32_t k = k0; k < k1; ++k) run += cnt[k];
   uint32_t total;
-  uint32_t acc = s0 + block_exclusive_scan(run, &total);
+  uint32_t acc = s0 + block_exclusive_scan<PERM_BLOCK>(run, &total);
   for (uint32_t k = k0; k < k1; ++k) {
     const uint32_t c = cnt[k];
     cnt[k] = acc;
     acc += c;
   }
   __syncthreads();
-  for (uint32_t i = s0 + threadIdx.x; i < s1; i += BIN_BLOCK) perm[atomicAdd(&cnt[node[i] - (uint32_t)lo], 1u)] = i;
+  for (uint32_t i = s0 + threadIdx.x; i < s1; i += PERM_BLOCK) perm[atomicAdd(&cnt[node[i] - (uint32_t)lo], 1u)] = i;
 }
 
 // The learned proposal for the listed points only (those in polyhedron
@@ -533,14 +545,16 @@ static brille_aux_out offset_aux(const brille_aux_out& x, uint64_t c0) {
   return o;
 }
 
-// buckets: enough that one bucket's share of the payload is about 64 KiB
-static uint32_t bucket_count(const brille_grid* G, int precision) {
-  const DataConst& d = G->d;
-  const uint64_t row = (uint64_t)(precision == BRILLE_FP32 ? d.stride32 * 4 : d.stride64 * 8);
-  const uint64_t payload = (uint64_t)d.V * d.B * row;
+// Buckets of about kNodesPerBucket trellis nodes: k_bucket_perm orders each
+// bucket by node, so the buckets only have to be few enough that a bucketing
+// block (BIN_SPAN points) puts several points into each, and the scatter
+// writes runs of records rather than single ones.
+constexpr uint32_t kNodesPerBucket = 1024;
+static uint32_t bucket_count(const brille_grid* G) {
+  const uint64_t n_nodes = std::max<uint64_t>(G->g.n_nodes, 1);
   uint32_t nb = 16;
-  while (nb < 2048 && (uint64_t)nb * 65536 < payload) nb <<= 1;
-  return (uint32_t)std::min<uint64_t>(nb, std::max<uint64_t>(G->g.n_nodes, 1));
+  while (nb < 2048 && (uint64_t)nb * kNodesPerBucket < n_nodes) nb <<= 1;
+  return (uint32_t)std::min<uint64_t>(nb, n_nodes);
 }
 
 bool binned_applies(const brille_grid* G, const LaunchArgs& a, int kernel) {
@@ -572,7 +586,7 @@ int launch_binned(brille_grid* G, const LaunchArgs& a, int precision, hipStream_
       if (hipEventCreate(&e) != hipSuccess) return fail(BRILLE_ERR_DEVICE, "hipEventCreate failed");
   hipEvent_t* evp = stage_ms ? ev : nullptr;
   const uint64_t chunk = std::min<uint64_t>(a.n, BIN_CHUNK);
-  const uint32_t nb = bucket_count(G, precision);
+  const uint32_t nb = bucket_count(G);
   const uint32_t nblk_max = (uint32_t)((chunk + BIN_SPAN - 1) / BIN_SPAN);
   const uint64_t m_max = (uint64_t)(nb + 1) * nblk_max;
   const uint64_t np_max = (m_max + SCAN_SEG - 1) / SCAN_SEG;
@@ -621,7 +635,7 @@ int launch_binned(brille_grid* G, const LaunchArgs& a, int precision, hipStream_
                          pcount, phint);
     hipLaunchKernelGGL(k_bin_scatter, dim3(nblk), dim3(BIN_BLOCK), 2 * lds1, s, rec, m, hist, nblk, nb,
                        G->g.n_nodes, srt, snode, phint);
-    hipLaunchKernelGGL(k_bucket_perm, dim3(nb + 1), dim3(BIN_BLOCK), 0, s, snode, m, hist, nblk, nb, G->g.n_nodes,
+    hipLaunchKernelGGL(k_bucket_perm, dim3(nb + 1), dim3(PERM_BLOCK), 0, s, snode, m, hist, nblk, nb, G->g.n_nodes,
                        perm);
     if (evp) (void)hipEventRecord(evp[2], s);
     hipLaunchKernelGGL(sums, dim3((m + TILE - 1) / TILE), dim3(TILE), 0, s, G->g, G->d, a.prop, srt, perm, m,
