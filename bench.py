@@ -167,12 +167,24 @@ def main():
     bytes_per_q = 24 + B * sv * 4 + B * sc * 8
     tb = table_bytes(T)
     alg = n * bytes_per_q + tb
+    # The path is one pipeline of kernels per call (fold, bucketing, sums): the
+    # algorithmic bytes of the call (Q in, outputs out, tables once) over the
+    # HIP-event time of all its kernels.  The largest stage is the sums kernel.
     achieved = alg / (k_fast * 1e-3) / 1e9
-    per_q = measured_traffic(cfg)
+    per_q = measured_traffic(cfg, head_name)
+    sums_ms = head["stages"].get("k_bin_sums")
     roofline = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
                     traffic=(per_q * n if per_q else None), bytes_per_q=bytes_per_q, table_bytes=tb,
-                    traffic_bytes_per_q=per_q, timed=f"{head_name}: all kernels of one call (HIP events)",
-                    kernel="k_bin_sums<float, Gamma layout> (largest stage)", stages=head["stages"])
+                    traffic_bytes_per_q=per_q,
+                    timed=f"{head_name}: all kernels of one call, HIP events on the call's stream",
+                    kernel="binned pipeline (k_bin_fold, k_scan_*, k_bin_scatter, k_bucket_perm, "
+                           "k_bin_propose when learned, k_bin_sums)",
+                    dominant_kernel=dict(name="k_bin_sums<float, Gamma layout>", ms=sums_ms,
+                                         share=(sums_ms / k_fast if sums_ms else None),
+                                         output_bytes_per_q=B * sv * 4 + B * sc * 8,
+                                         output_gbs=(n * (B * sv * 4 + B * sc * 8) / (sums_ms * 1e-3) / 1e9
+                                                     if sums_ms else None)),
+                    stages=head["stages"])
     cpu = None if args.no_cpu else cpu_baseline(T, cfg, c, args.cpu_seconds)
     out = {
         "metric": "Q-points/s ir_interpolate_at, 1 MI355X: learned+batched vs plain, %HBM peak",
@@ -223,17 +235,42 @@ def bench_batch(args, capi, S):
     sch = batch.schedule()
     log(f"autotuned schedule: {sch}")
     for _ in range(args.warmup):
-        batch.time(dQ, offsets, vals, vecs, precision=capi.FP32, iters=1)
+        capi.check(batch.run(dQ, offsets, vals, vecs, precision=capi.FP32))
+    capi.check(lib.brille_device_sync())
+    # a step is one batch call as a user makes it (slots staged, graph replayed,
+    # error records checked); the graph replay alone is timed with HIP events
     w0 = time.perf_counter()
-    kms = [batch.time(dQ, offsets, vals, vecs, precision=capi.FP32, iters=1) for _ in range(args.steps)]
+    for _ in range(args.steps):
+        capi.check(batch.run(dQ, offsets, vals, vecs, precision=capi.FP32))
     capi.check(lib.brille_device_sync())
     wall = (time.perf_counter() - w0) * 1000.0 / args.steps
+    kms = [batch.time(dQ, offsets, vals, vecs, precision=capi.FP32, iters=1) for _ in range(max(1, args.steps // 2))]
     seq = batch.time(dQ, offsets, vals, vecs, precision=capi.FP32, kernel=capi.KERNEL_TILE, iters=max(1, args.steps))
     plain = None
     if not args.no_plain:
-        v64, c64 = batch.outputs(offsets, capi.FP64)
-        plain = batch.time(dQ, offsets, v64, c64, precision=capi.FP64, kernel=capi.KERNEL_PLAIN, iters=1)
-        del v64, c64
+        # FP64 outputs of all 64 grids do not fit one device next to the FP32
+        # ones (config 2: 152 GB FP32, 304 GB FP64): the plain baseline runs
+        # grid by grid into one buffer sized for the largest grid
+        del vals, vecs
+        counts = np.diff(offsets.astype(np.int64))
+        vs_max = max(int(np.prod(devs[g].out_shapes(int(counts[g]))[0])) for g in range(len(devs)))
+        cs_max = max(int(np.prod(devs[g].out_shapes(int(counts[g]))[1])) for g in range(len(devs)))
+        dv64 = capi.DeviceArray((max(vs_max, 1),), np.float64)
+        dc64 = capi.DeviceArray((max(cs_max, 1),), np.complex128)
+        stream = C.c_void_p()
+        capi.check(lib.brille_stream_create(C.byref(stream)))
+        o = capi.RunOpts(0, capi.FP64, 0, capi.MEM_DEVICE, capi.KERNEL_PLAIN, 0)
+        plain = 0.0
+        for g, dev in enumerate(devs):
+            if counts[g] == 0:
+                continue
+            t = capi.KernelTimes()
+            qp = C.c_void_p(dQ.ptr.value + int(offsets[g]) * 24)
+            capi.check(lib.brille_time_ir_interpolate_at(dev.h, qp, int(counts[g]), C.byref(o), dv64.ptr, dc64.ptr, 1,
+                                                         stream, C.byref(t)))
+            plain += float(t.total_ms)
+        capi.check(lib.brille_stream_destroy(stream))
+        del dv64, dc64
     log(f"batched fp32: {wall:.3f} ms/step wall, graph {np.mean(kms):.3f} ms; per-grid tile calls {seq:.3f} ms; "
         f"per-grid plain fp64 {plain if plain is None else round(plain, 3)} ms")
     per_q = np.zeros(len(devs))
@@ -300,15 +337,17 @@ def batch_cpu_baseline(tables, Q, offsets, seconds):
                        f"FP64, {cores} OpenMP threads")
 
 
-def measured_traffic(cfg):
-    """HBM bytes per Q of the headline kernel, from the committed rocprofv3 --pmc
-    summary (FETCH_SIZE + WRITE_SIZE per dispatch over Q per dispatch, scaled by
-    the copy-kernel calibration); None when no summary exists for this config."""
-    path = os.path.join(ROOT, "profiles", f"hbm_traffic_k_ir_interpolate_config{cfg}.md")
+def measured_traffic(cfg, path_name):
+    """HBM bytes per Q of one call of the headline path, from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE of every kernel of the
+    call, scaled by the copy-kernel calibration, over Q); None when there is no
+    summary for this config and path."""
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_binned_config{cfg}.md")
     if not os.path.exists(path):
         return None
+    key = "traffic_bytes_per_q[" + path_name + "]:"
     for line in open(path):
-        if line.startswith("traffic_bytes_per_q:"):
+        if line.startswith(key):
             return float(line.split(":", 1)[1])
     return None
 

@@ -99,9 +99,11 @@ __device__ __forceinline__ void batch_tile(const BatchGrid& bg, const BatchSlot&
   __syncthreads();
 }
 
+// FP32: three waves per SIMD (the wide-row branch needs the occupancy; one
+// spilled dword)
 template <typename Real, int TB>
-__global__ __launch_bounds__(TB) void k_batch(const BatchGrid* grids, const BatchSlot* slots, BatchHeader* hdr,
-                                              uint32_t grab) {
+__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(sizeof(Real) == 4 ? 3 : 1)))
+void k_batch(const BatchGrid* grids, const BatchSlot* slots, BatchHeader* hdr, uint32_t grab) {
   using C = typename Cplx<Real>::T;
   __shared__ PointRec<Real> s_rec[TB];
   __shared__ C s_ph[TB * 4];

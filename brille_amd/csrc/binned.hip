@@ -300,6 +300,18 @@ __global__ __launch_bounds__(256) void k_bin_propose(GridConst g, ProposalConst 
   }
 }
 
+// XCD-aware tile order: the hardware hands consecutive blocks to the eight
+// XCDs in turn, each with its own L2.  Block b runs on XCD b % 8, so giving XCD
+// x the contiguous tiles [start(x), start(x) + count(x)) keeps a bucket's
+// records and rows in one L2 instead of fetching them into all eight.  The
+// map is a bijection for any grid size.
+constexpr uint32_t kXcds = 8;
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t nb) {
+  const uint32_t q = nb / kXcds, r = nb % kXcds;
+  const uint32_t x = b % kXcds, i = b / kXcds;
+  return x * q + (x < r ? x : r) + i;
+}
+
 template <typename Real, int L, int SC, int SV, bool Prop>
 __global__ __launch_bounds__(TILE) void k_bin_sums(GridConst g, DataConst d, ProposalConst pc, const SortRec* srt,
                                                     const uint32_t* perm, uint32_t n, void* __restrict__ vals,
@@ -317,7 +329,7 @@ __global__ __launch_bounds__(TILE) void k_bin_sums(GridConst g, DataConst d, Pro
     s_tried = 0;
     s_verified = 0;
   }
-  const uint32_t tile0 = blockIdx.x * TILE;
+  const uint32_t tile0 = xcd_tile(blockIdx.x, gridDim.x) * TILE;
   const uint32_t tid = threadIdx.x;
   const uint32_t p = tile0 + tid;
   const bool want_phase = d.vec_rot == BRILLE_ROT_GAMMA && d.ec[1] > 0;

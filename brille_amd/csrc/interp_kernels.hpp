@@ -849,10 +849,19 @@ __device__ __forceinline__ double wave_sum_f(double v) {
 // row is read with coalesced loads; the Hermitian product is a wave sum.  The
 // Gamma rotation mixes the three elements of an atom, so the accumulated row
 // goes through the wave's LDS slice `lds` (kWideMax elements).
+// What a lane's Γ rotation needs of a point, kept while a wave stays on it
+template <typename Real>
+struct WideCache {
+  typename Cplx<Real>::T ph[2];
+  Real R[2][3];
+  uint32_t dst[2];
+};
+
+// fresh: the wave's previous unit was another point (reload the cache)
 template <typename Real>
 __device__ inline void unit_wide(const GridConst& g, const DataConst& d, const PointRec<Real>& r, const double* q,
                                  uint32_t b, Real* ov, typename Cplx<Real>::T* oc, typename Cplx<Real>::T* lds,
-                                 bool gamma, bool cart) {
+                                 bool gamma, bool cart, WideCache<Real>& wc, bool fresh) {
   using C = typename Cplx<Real>::T;
   const uint32_t l = threadIdx.x & 63u;
   const uint32_t sc = d.sc, sv = d.sv;
@@ -860,29 +869,48 @@ __device__ inline void unit_wide(const GridConst& g, const DataConst& d, const P
   const C zero = Elem<C>::zero();
   const C* prow = reinterpret_cast<const C*>(row_at<Real>(d, r.pivot, b));
   const C p0 = h0 ? prow[l] : zero, p1 = h1 ? prow[l + 64] : zero;
-  C a0 = zero, a1 = zero;
-  Real va = 0;
-  const int nslot = r.nslot;
+  // All eight rows are requested first and the sixteen Hermitian wave sums
+  // run side by side.  Unused tetrahedron slots repeat the pivot with weight
+  // 0 and add exact zeros.
+  C v0[8], v1[8];
+  Real vv[8];
 #pragma unroll
   for (int x = 0; x < 8; ++x) {
-    if (x >= nslot) break;  // wave-uniform: one point per wave
-    const Real w = r.w[x];
     const Real* row = row_at<Real>(d, r.idx[x], b);
     const C* rc = reinterpret_cast<const C*>(row);
-    const C v0 = h0 ? rc[l] : zero, v1 = h1 ? rc[l + 64] : zero;
-    Real re = p0.x * v0.x + p0.y * v0.y;
-    re += p1.x * v1.x + p1.y * v1.y;
-    Real im = p0.x * v0.y - p0.y * v0.x;
-    im += p1.x * v1.y - p1.y * v1.x;
-    re = wave_sum_f(re);
-    im = wave_sum_f(im);
-    const C e = antiphase_closed<Real>(re, im);
-    const Real cx = w * e.x, cy = w * e.y;
-    a0.x += cx * v0.x - cy * v0.y;
-    a0.y += cx * v0.y + cy * v0.x;
-    a1.x += cx * v1.x - cy * v1.y;
-    a1.y += cx * v1.y + cy * v1.x;
-    if (l < sv) va += w * row[2 * sc + l];
+    v0[x] = h0 ? rc[l] : zero;
+    v1[x] = h1 ? rc[l + 64] : zero;
+    vv[x] = l < sv ? row[2 * sc + l] : (Real)0;
+  }
+  Real re[8], im[8];
+#pragma unroll
+  for (int x = 0; x < 8; ++x) {
+    re[x] = p0.x * v0[x].x + p0.y * v0[x].y;
+    re[x] += p1.x * v1[x].x + p1.y * v1[x].y;
+    im[x] = p0.x * v0[x].y - p0.y * v0[x].x;
+    im[x] += p1.x * v1[x].y - p1.y * v1[x].x;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+    for (int x = 0; x < 8; ++x) {
+      re[x] += __shfl_xor(re[x], o);
+      im[x] += __shfl_xor(im[x], o);
+    }
+  C a0 = zero, a1 = zero;
+  Real va = 0;
+#pragma unroll
+  for (int x = 0; x < 8; ++x) {
+    const Real w = r.w[x];
+    const C e = antiphase_closed<Real>(re[x], im[x]);
+    Real px, py;
+    phase_elem<Real>(e, v0[x].x, v0[x].y, &px, &py);
+    a0.x = fma_r(w, px, a0.x);
+    a0.y = fma_r(w, py, a0.y);
+    phase_elem<Real>(e, v1[x].x, v1[x].y, &px, &py);
+    a1.x = fma_r(w, px, a1.x);
+    a1.y = fma_r(w, py, a1.y);
+    va = fma_r(w, vv[x], va);
   }
   if (l < sv) ov[l] = va;
   if (!gamma) {
@@ -901,19 +929,19 @@ __device__ inline void unit_wide(const GridConst& g, const DataConst& d, const P
     const uint32_t el = l + 64u * h;
     if (el < sc) {
       const uint32_t a = el / 3u, i = el - 3u * a;
+      if (fresh) {
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+          wc.R[h][j] = cart ? (Real)d.cart_rot[9 * (uint64_t)r.invrot + 3 * i + j]
+                            : (Real)g.rot[9 * (uint64_t)r.invrot + 3 * i + j];
+        wc.ph[h] = gamma_phase<Real>(d, q, r.invrot, a);
+        wc.dst[h] = 3u * d.F0[a * d.gamma_ops + r.invrot] + i;
+      }
       const C v[3] = {lds[3 * a], lds[3 * a + 1], lds[3 * a + 2]};
       C t = zero;
-      if (cart) {
-        const double* R = d.cart_rot + 9 * (uint64_t)r.invrot + 3 * i;
 #pragma unroll
-        for (int j = 0; j < 3; ++j) t = Elem<C>::add(t, Elem<C>::scale(v[j], (Real)R[j]));
-      } else {
-        const int32_t* R = g.rot + 9 * (uint64_t)r.invrot + 3 * i;
-#pragma unroll
-        for (int j = 0; j < 3; ++j) t = Elem<C>::add(t, Elem<C>::scale(v[j], (Real)R[j]));
-      }
-      const C ph = gamma_phase<Real>(d, q, r.invrot, a);
-      oc[3u * d.F0[a * d.gamma_ops + r.invrot] + i] = cmul(ph, t);
+      for (int j = 0; j < 3; ++j) t = Elem<C>::add(t, Elem<C>::scale(v[j], wc.R[h][j]));
+      oc[wc.dst[h]] = cmul(wc.ph[h], t);
     }
   }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -1154,6 +1182,8 @@ __device__ __forceinline__ void tile_sums(const GridConst& g, const DataConst& d
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
     const bool gamma = d.vec_rot == BRILLE_ROT_GAMMA && d.ec[1] > 0;
     C* lds = s_wide + wave * kWideMax;
+    WideCache<Real> wc;
+    uint32_t last = 0xFFFFFFFFu;
     for (uint32_t u = wave; u < units; u += TB / 64) {
       const uint32_t pl = u / B, b = u - pl * B;
       const PointRec<Real>& r = s_rec[pl];
@@ -1165,7 +1195,8 @@ __device__ __forceinline__ void tile_sums(const GridConst& g, const DataConst& d
         for (uint32_t e = lane; e < d.sc; e += 64) oc[e] = Elem<C>::zero();
         continue;
       }
-      unit_wide<Real>(g, d, r, s_q + 3 * pl, b, ov, oc, lds, gamma, cart);
+      unit_wide<Real>(g, d, r, s_q + 3 * pl, b, ov, oc, lds, gamma, cart, wc, pl != last);
+      last = pl;
     }
     return;
   }
