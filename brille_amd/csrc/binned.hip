@@ -267,6 +267,18 @@ __global__ __launch_bounds__(256) void k_bin_propose(GridConst g, ProposalConst 
                                                      const uint32_t* list, const uint32_t* count, uint32_t* hint) {
   __shared__ float s_feat[4][48];
   __shared__ uint32_t s_t0[4][16], s_nc[4][16], s_out[4][16];
+  // the first two layers' weights, held by the block for all its chunks
+  __shared__ float s_w1[3 * kPropH], s_b1[kPropH], s_b2[kPropH];
+  __shared__ __attribute__((aligned(16))) uint16_t s_w2[kPropH * kPropH];
+  for (uint32_t i = threadIdx.x; i < 3 * kPropH; i += 256) s_w1[i] = pc.W1[i];
+  for (uint32_t i = threadIdx.x; i < kPropH; i += 256) {
+    s_b1[i] = pc.b1[i];
+    s_b2[i] = pc.b2[i];
+  }
+  for (uint32_t i = threadIdx.x; i < kPropH * kPropH / 8; i += 256)
+    reinterpret_cast<uint4*>(s_w2)[i] = reinterpret_cast<const uint4*>(pc.W2)[i];
+  __syncthreads();
+  const PropWeights<LdsF, LdsH> W{(LdsF)s_w1, (LdsF)s_b1, (LdsF)s_b2, (LdsH)s_w2};
   const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
   const uint32_t n = *count;
   const uint32_t waves = gridDim.x * 4u;
@@ -289,7 +301,7 @@ __global__ __launch_bounds__(256) void k_bin_propose(GridConst g, ProposalConst 
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    propose16(pc, s_feat[wave], s_t0[wave], s_nc[wave], k, s_out[wave]);
+    propose16(pc, W, s_feat[wave], s_t0[wave], s_nc[wave], k, s_out[wave]);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");

@@ -265,7 +265,8 @@ __global__ __launch_bounds__(256) void k_prop_predict(ProposalConst pc, const fl
   const uint64_t c0 = (uint64_t)wave * 16;
   if (c0 >= n) return;  // whole waves leave together
   const uint32_t k = n - c0 < 16 ? (uint32_t)(n - c0) : 16u;
-  propose16(pc, feat + 3 * c0, t0 + c0, nc + c0, k, out + c0);
+  const PropWeights<const float*, const uint16_t*> W{pc.W1, pc.b1, pc.b2, pc.W2};
+  propose16(pc, W, feat + 3 * c0, t0 + c0, nc + c0, k, out + c0);
 }
 
 __global__ void k_count_equal(const uint32_t* a, const int32_t* b, uint32_t n, unsigned int* hits) {

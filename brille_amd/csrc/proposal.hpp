@@ -39,13 +39,28 @@ __device__ __forceinline__ void prop_features(const GridConst& g, const double* 
   }
 }
 
+// Layers 1 and 2 read W1, b1, W2 and b2 through W (global memory, or LDS
+// copies held by the block, which keep them out of the vector memory path).
+template <typename F, typename H>
+struct PropWeights {
+  F W1, b1, b2;
+  H W2;
+};
+using LdsF = const __attribute__((address_space(3))) float*;
+using LdsH = const __attribute__((address_space(3))) uint16_t*;
+__device__ __forceinline__ bf16x8_t load_bf16x8(const uint16_t* p) { return *reinterpret_cast<const bf16x8_t*>(p); }
+__device__ __forceinline__ bf16x8_t load_bf16x8(LdsH p) {
+  return *reinterpret_cast<const __attribute__((address_space(3))) bf16x8_t*>(p);
+}
+
 // Proposals for the n <= 16 points of a chunk (feat: 3 floats per point,
 // t0/nc: first candidate and candidate count).  Called by a whole wave with
 // wave-uniform arguments; out[i] receives the first candidate of largest logit.
 // Out of line: inlined into a loop over chunks, its per-lane layer-1 weights
 // would be hoisted and held across iterations (over a hundred registers).
-__device__ __noinline__ void propose16(const ProposalConst& pc, const float* feat, const uint32_t* t0, const uint32_t* nc,
-                                 uint32_t n, uint32_t* out) {
+template <typename F, typename H>
+__device__ __noinline__ void propose16(const ProposalConst& pc, PropWeights<F, H> W, const float* feat,
+                                       const uint32_t* t0, const uint32_t* nc, uint32_t n, uint32_t* out) {
   const uint32_t l = threadIdx.x & 63u, m = l & 15u, grp = l >> 4;
   float x0 = 0.f, x1 = 0.f, x2 = 0.f;
   if (m < n) {
@@ -59,20 +74,21 @@ __device__ __noinline__ void propose16(const ProposalConst& pc, const float* fea
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
       const int k = kb * 32 + 8 * (int)grp + e;
-      float h = pc.W1[3 * k] * x0;
-      h += pc.W1[3 * k + 1] * x1;
-      h += pc.W1[3 * k + 2] * x2;
-      h += pc.b1[k];
+      float h = W.W1[3 * k] * x0;
+      h += W.W1[3 * k + 1] * x1;
+      h += W.W1[3 * k + 2] * x2;
+      h += W.b1[k];
       a[kb][e] = (__bf16)(h > 0.f ? h : 0.f);
     }
   floatx4_t acc[8];
 #pragma unroll
   for (int nb = 0; nb < 8; ++nb) {
     acc[nb] = floatx4_t{0.f, 0.f, 0.f, 0.f};
-    const bf16x8_t* w = reinterpret_cast<const bf16x8_t*>(pc.W2 + (size_t)(nb * 16 + m) * kPropH + 8 * grp);
+    const uint32_t w0 = (nb * 16 + m) * kPropH + 8 * grp;
 #pragma unroll
-    for (int kb = 0; kb < 4; ++kb) acc[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[kb], w[kb * 4], acc[nb], 0, 0, 0);
-    const float bj = pc.b2[nb * 16 + m];
+    for (int kb = 0; kb < 4; ++kb)
+      acc[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[kb], load_bf16x8(W.W2 + w0 + 32 * kb), acc[nb], 0, 0, 0);
+    const float bj = W.b2[nb * 16 + m];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const float v = acc[nb][r] + bj;
@@ -131,57 +147,6 @@ __device__ __noinline__ void propose16(const ProposalConst& pc, const float* fea
       if (pi < n) out[pi] = bestc[r];
     }
   }
-}
-
-// LDS queue of the points of one tile that need the surface search
-template <int TB>
-struct PropQueue {
-  uint32_t count, tried, verified, pad;
-  float feat[3 * TB];
-  uint32_t t0[TB], nc[TB], out[TB];
-};
-
-// Block-wide: compacts the lanes with `want` into the queue (wave ballot and
-// prefix count, one LDS atomic per wave), runs the network over the queue 16
-// points per wave, and returns the proposal of the calling lane (kNoTet when
-// it did not want one).  Every thread of the block must call it.
-template <int TB>
-__device__ inline uint32_t propose_block(const GridConst& g, const ProposalConst& pc, PropQueue<TB>& pq, bool want,
-                                         const double* q, uint32_t node) {
-  if (threadIdx.x == 0) pq.count = 0;
-  __syncthreads();
-  const uint32_t lane = threadIdx.x & 63u;
-  const unsigned long long mask = __ballot(want);
-  const uint32_t rank = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
-  uint32_t base = 0;
-  if (mask) {
-    const uint32_t leader = (uint32_t)__ffsll((long long)mask) - 1u;
-    if (lane == leader) base = atomicAdd(&pq.count, (uint32_t)__popcll(mask));
-    base = __shfl(base, (int)leader);
-  }
-  const uint32_t slot = base + rank;
-  if (want) {
-    double x[3];
-    mat_vec(x, g.outer_to_xyz, q);
-    float f[3];
-    prop_features(g, x, node, f);
-    pq.feat[3 * slot] = f[0];
-    pq.feat[3 * slot + 1] = f[1];
-    pq.feat[3 * slot + 2] = f[2];
-    const uint32_t p = g.node_index[node];
-    const uint32_t a = g.poly_off[p];
-    pq.t0[slot] = a;
-    pq.nc[slot] = g.poly_off[p + 1] - a;
-  }
-  __syncthreads();
-  const uint32_t n = pq.count;
-  const uint32_t wave = threadIdx.x >> 6;
-  for (uint32_t c0 = wave * 16u; c0 < n; c0 += (TB / 64) * 16u) {
-    const uint32_t k = n - c0 < 16u ? n - c0 : 16u;
-    propose16(pc, pq.feat + 3 * c0, pq.t0 + c0, pq.nc + c0, k, pq.out + c0);
-  }
-  __syncthreads();
-  return want ? pq.out[slot] : kNoTet;
 }
 
 }  // namespace brille_amd
