@@ -178,6 +178,8 @@ struct brille_batch {
   hipStream_t cap_stream = nullptr;
   // instantiated graphs by (precision, tile, blocks, grab)
   std::map<std::tuple<int, uint32_t, uint32_t, uint32_t>, hipGraphExec_t> graphs;
+  // host copy of d_grids, refreshed when a grid's tables change
+  std::vector<brille_amd::BatchGrid> host_grids;
 };
 
 namespace brille_amd {
@@ -306,9 +308,35 @@ static int run_sequential(brille_batch* b, const double* Q, const uint64_t* offs
   return BRILLE_OK;
 }
 
+// The wide factor tables of the wide-row grids (built on first use, the same
+// tables their own binned calls use); the grid copies on the device follow.
+static int ensure_batch_tables(brille_batch* b, int precision, hipStream_t s) {
+  bool changed = false;
+  for (size_t g = 0; g < b->grids.size(); ++g) {
+    brille_grid* G = b->grids[g];
+    if (!wide_rows(G->d)) continue;
+    const void* before = precision == BRILLE_FP32 ? (const void*)G->d.wide_fac32 : (const void*)G->d.wide_fac64;
+    const int rc = ensure_cube_rows(G, precision, s);
+    if (rc != BRILLE_OK) return rc;
+    const void* after = precision == BRILLE_FP32 ? (const void*)G->d.wide_fac32 : (const void*)G->d.wide_fac64;
+    if (after != before) {
+      b->host_grids[g].d = G->d;
+      changed = true;
+    }
+  }
+  if (changed) {
+    BATCH_TRY(hipStreamSynchronize(s));
+    BATCH_TRY(hipMemcpy(b->d_grids, b->host_grids.data(), b->host_grids.size() * sizeof(BatchGrid),
+                        hipMemcpyHostToDevice));
+  }
+  return BRILLE_OK;
+}
+
 static int run_with(brille_batch* b, const double* Q, const uint64_t* offsets, const brille_run_opts& o,
                     void* const* vals, void* const* vecs, hipStream_t s, const Schedule& sc) {
-  int rc = stage_slots(b, Q, offsets, vals, vecs, sc, o.no_move ? FOLD_NONE : FOLD_IR, s);
+  int rc = ensure_batch_tables(b, o.precision, s);
+  if (rc != BRILLE_OK) return rc;
+  rc = stage_slots(b, Q, offsets, vals, vecs, sc, o.no_move ? FOLD_NONE : FOLD_IR, s);
   if (rc != BRILLE_OK) return rc;
   hipGraphExec_t exec;
   rc = graph_for(b, o.precision, sc, &exec);
@@ -360,6 +388,7 @@ int brille_batch_create(brille_grid* const* grids, uint32_t n_grids, brille_batc
     return fail(BRILLE_ERR_DEVICE, "device allocation for the batch failed");
   }
   b->d_grids = static_cast<BatchGrid*>(pg);
+  b->host_grids = host;
   b->d_slots = static_cast<BatchSlot*>(ps);
   b->d_hdr = static_cast<BatchHeader*>(ph);
   b->cap_slots = n_grids;

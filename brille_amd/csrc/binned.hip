@@ -455,6 +455,52 @@ __global__ __launch_bounds__(256) void k_table_rows(GridConst g, DataConst d, ui
   for (uint32_t k = 2 * SC; k < stride; ++k) dst[k] = src[k];  // values and zero padding
 }
 
+// The wide factor table: wave w = (block row, mode b) computes the antiphases
+// of a cube's eight or a tetrahedron's four slot rows against its slot-0 row
+// with the sums' own wave-wide function (same lanes, same reductions), so a
+// point summed from the table gets the same bits.
+template <typename Real>
+__global__ __launch_bounds__(256) void k_wide_factors(GridConst g, DataConst d, uint64_t n_cube_units,
+                                                      uint64_t n_units, typename Cplx<Real>::T* out) {
+  using C = typename Cplx<Real>::T;
+  const uint64_t t = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (t >= n_units) return;  // wave-uniform
+  const uint32_t l = threadIdx.x & 63u;
+  const uint32_t sc = d.sc;
+  const bool h0 = l < sc, h1 = l + 64u < sc;
+  uint32_t v[8], ns, b;
+  uint64_t trow;
+  if (t < n_cube_units) {
+    const uint64_t c = t / d.B;
+    b = (uint32_t)(t - c * d.B);
+#pragma unroll
+    for (int x = 0; x < 8; ++x) v[x] = g.cube_v[8 * c + 7 - x];
+    ns = 8;
+    trow = c * 8 * d.B;
+  } else {
+    const uint64_t tt = (t - n_cube_units) / d.B;
+    b = (uint32_t)(t - n_cube_units - tt * d.B);
+#pragma unroll
+    for (int x = 0; x < 8; ++x) v[x] = g.tet_v[4 * tt + (x < 4 ? x : 0)];
+    ns = 4;
+    trow = d.tab_tet_row + tt * 4 * d.B;
+  }
+  const C zero = Elem<C>::zero();
+  C v0[8], v1[8], e[8];
+#pragma unroll
+  for (int x = 0; x < 8; ++x) {
+    const C* rc = reinterpret_cast<const C*>(row_at<Real>(d, v[x], b));
+    v0[x] = h0 ? rc[l] : zero;
+    v1[x] = h1 ? rc[l + 64] : zero;
+  }
+  wide_antiphases<Real, 8>(v0[0], v1[0], v0, v1, e);
+  if (l == 0) {
+#pragma unroll
+    for (int x = 0; x < 8; ++x)
+      if ((uint32_t)x < ns) out[trow + (uint64_t)x * d.B + b] = e[x];
+  }
+}
+
 using SumsKernel = void (*)(GridConst, DataConst, ProposalConst, const SortRec*, const uint32_t*, uint32_t, void*,
                            void*, brille_aux_out, ErrRec*, uint64_t);
 
@@ -528,6 +574,24 @@ static int build_cube_rows(brille_grid* G, hipStream_t s, Real** out) {
   return BRILLE_OK;
 }
 
+template <typename Real>
+static int build_wide_factors(brille_grid* G, hipStream_t s, typename Cplx<Real>::T** out) {
+  using C = typename Cplx<Real>::T;
+  const DataConst& d = G->d;
+  const uint64_t cube_units = G->n_cubes * d.B;
+  const uint64_t units = cube_units + G->n_tets * d.B;
+  void* p = nullptr;
+  if (hipMalloc(&p, (G->n_cubes * 8 + G->n_tets * 4) * d.B * sizeof(C)) != hipSuccess)
+    return fail(BRILLE_ERR_DEVICE, "hipMalloc failed for the wide factor table");
+  G->data_allocs.push_back(p);
+  C* t = static_cast<C*>(p);
+  hipLaunchKernelGGL((k_wide_factors<Real>), dim3((unsigned)((units + 3) / 4)), dim3(256), 0, s, G->g, d, cube_units,
+                     units, t);
+  if (hipGetLastError() != hipSuccess) return fail(BRILLE_ERR_DEVICE, "wide factor table launch failed");
+  *out = t;
+  return BRILLE_OK;
+}
+
 // Builds the table of the given precision once per payload (freed with it),
 // eight rows per cube: payloads whose sums are not the one-pass kind, grids
 // without cubes and tables beyond kCubeRowsMax go without, and their sums
@@ -535,7 +599,26 @@ static int build_cube_rows(brille_grid* G, hipStream_t s, Real** out) {
 constexpr uint64_t kCubeRowsMax = (4ull << 30) - 1;  // the sums address the table with 32-bit offsets
 int ensure_cube_rows(brille_grid* G, int precision, hipStream_t s) {
   DataConst& d = G->d;
-  if (G->n_cubes + G->n_tets == 0 || d.B == 0 || onepass_sc(d) == 0) return BRILLE_OK;
+  if (G->n_cubes + G->n_tets == 0 || d.B == 0) return BRILLE_OK;
+  if (wide_rows(d)) {
+    // wide rows: the antiphases only (one complex per slot row)
+    const uint64_t fac_bytes = (G->n_cubes * 8 + G->n_tets * 4) * d.B * (precision == BRILLE_FP32 ? 8ull : 16ull);
+    if (fac_bytes > kCubeRowsMax) return BRILLE_OK;
+    d.tab_tet_row = (uint32_t)(G->n_cubes * 8 * d.B);
+    if (precision == BRILLE_FP32) {
+      if (d.wide_fac32) return BRILLE_OK;
+      float2* t = nullptr;
+      const int rc = build_wide_factors<float>(G, s, &t);
+      if (rc == BRILLE_OK) d.wide_fac32 = t;
+      return rc;
+    }
+    if (d.wide_fac64) return BRILLE_OK;
+    double2* t = nullptr;
+    const int rc = build_wide_factors<double>(G, s, &t);
+    if (rc == BRILLE_OK) d.wide_fac64 = t;
+    return rc;
+  }
+  if (onepass_sc(d) == 0) return BRILLE_OK;
   const uint64_t row_bytes = precision == BRILLE_FP32 ? d.stride32 * 4ull : d.stride64 * 8ull;
   if ((G->n_cubes * 8 + G->n_tets * 4) * d.B * row_bytes > kCubeRowsMax) return BRILLE_OK;
   d.tab_tet_row = (uint32_t)(G->n_cubes * 8 * d.B);

@@ -97,6 +97,11 @@ struct DataConst {
   const float* cube_rows32;
   const double* cube_rows64;
   uint32_t tab_tet_row;
+  // Wide-row payloads instead keep only the antiphases: the factor of slot x
+  // for mode b of the block at table row trow (numbered as above) at
+  // trow + x * B + b (null until built).
+  const float2* wide_fac32;
+  const double2* wide_fac64;
   // GammaTable
   uint32_t n_atoms, gamma_ops;
   const uint32_t* F0;

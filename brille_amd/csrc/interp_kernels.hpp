@@ -849,6 +849,41 @@ __device__ __forceinline__ double wave_sum_f(double v) {
 // row is read with coalesced loads; the Hermitian product is a wave sum.  The
 // Gamma rotation mixes the three elements of an atom, so the accumulated row
 // goes through the wave's LDS slice `lds` (kWideMax elements).
+template <typename Real>
+__device__ __forceinline__ const typename Cplx<Real>::T* wide_factors(const DataConst& d);
+template <>
+__device__ __forceinline__ const float2* wide_factors<float>(const DataConst& d) { return d.wide_fac32; }
+template <>
+__device__ __forceinline__ const double2* wide_factors<double>(const DataConst& d) { return d.wide_fac64; }
+
+// Antiphases of NS slot rows against the pivot row for one wide-row unit,
+// computed by the whole wave: lane l holds complex elements l and l + 64 of
+// each row (p0/p1 of the pivot, v0/v1 of the slots); the Hermitian products
+// are wave sums, all slots side by side.  One definition for the sums and
+// for the factor table, so both give the same bits.
+template <typename Real, int NS>
+__device__ __forceinline__ void wide_antiphases(typename Cplx<Real>::T p0, typename Cplx<Real>::T p1,
+                                                const typename Cplx<Real>::T* v0, const typename Cplx<Real>::T* v1,
+                                                typename Cplx<Real>::T* e) {
+  Real re[NS], im[NS];
+#pragma unroll
+  for (int x = 0; x < NS; ++x) {
+    re[x] = p0.x * v0[x].x + p0.y * v0[x].y;
+    re[x] += p1.x * v1[x].x + p1.y * v1[x].y;
+    im[x] = p0.x * v0[x].y - p0.y * v0[x].x;
+    im[x] += p1.x * v1[x].y - p1.y * v1[x].x;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+    for (int x = 0; x < NS; ++x) {
+      re[x] += __shfl_xor(re[x], o);
+      im[x] += __shfl_xor(im[x], o);
+    }
+#pragma unroll
+  for (int x = 0; x < NS; ++x) e[x] = antiphase_closed<Real>(re[x], im[x]);
+}
+
 // What a lane's Γ rotation needs of a point, kept while a wave stays on it
 template <typename Real>
 struct WideCache {
@@ -857,18 +892,18 @@ struct WideCache {
   uint32_t dst[2];
 };
 
-// fresh: the wave's previous unit was another point (reload the cache)
+// fresh: the wave's previous unit was another point (reload the cache);
+// table: take the antiphases of points with a row-table block from the wide
+// factor table instead of the wave sums
 template <typename Real>
 __device__ inline void unit_wide(const GridConst& g, const DataConst& d, const PointRec<Real>& r, const double* q,
                                  uint32_t b, Real* ov, typename Cplx<Real>::T* oc, typename Cplx<Real>::T* lds,
-                                 bool gamma, bool cart, WideCache<Real>& wc, bool fresh) {
+                                 bool gamma, bool cart, WideCache<Real>& wc, bool fresh, bool table) {
   using C = typename Cplx<Real>::T;
   const uint32_t l = threadIdx.x & 63u;
   const uint32_t sc = d.sc, sv = d.sv;
   const bool h0 = l < sc, h1 = l + 64u < sc;
   const C zero = Elem<C>::zero();
-  const C* prow = reinterpret_cast<const C*>(row_at<Real>(d, r.pivot, b));
-  const C p0 = h0 ? prow[l] : zero, p1 = h1 ? prow[l + 64] : zero;
   // All eight rows are requested first and the sixteen Hermitian wave sums
   // run side by side.  Unused tetrahedron slots repeat the pivot with weight
   // 0 and add exact zeros.
@@ -882,32 +917,27 @@ __device__ inline void unit_wide(const GridConst& g, const DataConst& d, const P
     v1[x] = h1 ? rc[l + 64] : zero;
     vv[x] = l < sv ? row[2 * sc + l] : (Real)0;
   }
-  Real re[8], im[8];
+  C e[8];
+  if (table && r.trow != kNoCube) {
+    // wave-uniform branch: one unit per wave
+    const C* fac = wide_factors<Real>(d);
 #pragma unroll
-  for (int x = 0; x < 8; ++x) {
-    re[x] = p0.x * v0[x].x + p0.y * v0[x].y;
-    re[x] += p1.x * v1[x].x + p1.y * v1[x].y;
-    im[x] = p0.x * v0[x].y - p0.y * v0[x].x;
-    im[x] += p1.x * v1[x].y - p1.y * v1[x].x;
+    for (int x = 0; x < 8; ++x) e[x] = x < (int)r.nslot ? fac[r.trow + (uint32_t)x * d.B + b] : C{(Real)1, (Real)0};
+  } else {
+    const C* prow = reinterpret_cast<const C*>(row_at<Real>(d, r.pivot, b));
+    const C p0 = h0 ? prow[l] : zero, p1 = h1 ? prow[l + 64] : zero;
+    wide_antiphases<Real, 8>(p0, p1, v0, v1, e);
   }
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1)
-#pragma unroll
-    for (int x = 0; x < 8; ++x) {
-      re[x] += __shfl_xor(re[x], o);
-      im[x] += __shfl_xor(im[x], o);
-    }
   C a0 = zero, a1 = zero;
   Real va = 0;
 #pragma unroll
   for (int x = 0; x < 8; ++x) {
     const Real w = r.w[x];
-    const C e = antiphase_closed<Real>(re[x], im[x]);
     Real px, py;
-    phase_elem<Real>(e, v0[x].x, v0[x].y, &px, &py);
+    phase_elem<Real>(e[x], v0[x].x, v0[x].y, &px, &py);
     a0.x = fma_r(w, px, a0.x);
     a0.y = fma_r(w, py, a0.y);
-    phase_elem<Real>(e, v1[x].x, v1[x].y, &px, &py);
+    phase_elem<Real>(e[x], v1[x].x, v1[x].y, &px, &py);
     a1.x = fma_r(w, px, a1.x);
     a1.y = fma_r(w, py, a1.y);
     va = fma_r(w, vv[x], va);
@@ -1195,7 +1225,8 @@ __device__ __forceinline__ void tile_sums(const GridConst& g, const DataConst& d
         for (uint32_t e = lane; e < d.sc; e += 64) oc[e] = Elem<C>::zero();
         continue;
       }
-      unit_wide<Real>(g, d, r, s_q + 3 * pl, b, ov, oc, lds, gamma, cart, wc, pl != last);
+      unit_wide<Real>(g, d, r, s_q + 3 * pl, b, ov, oc, lds, gamma, cart, wc, pl != last,
+                      wide_factors<Real>(d) != nullptr);
       last = pl;
     }
     return;

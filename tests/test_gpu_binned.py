@@ -103,3 +103,26 @@ def test_binned_several_passes():
     assert rb == rt != 0
     np.testing.assert_array_equal(vb, vt)
     np.testing.assert_array_equal(cb, ct)
+
+
+def test_wide_rows_binned_equals_tile_and_oracle():
+    """A many-atom payload (rows of more than 12 complex elements: one wave per
+    (point, mode), antiphases from the wide factor table in the binned path)."""
+    grids, Q, offsets = S.batch_grids(n_grids=6, v_range=(800, 2000), n_q=20_000)
+    sc = [int(sum(g.tables()["vectors_elements"])) for g in grids]
+    gi = int(np.argmin([c if c > 12 else 1 << 30 for c in sc]))
+    assert sc[gi] > 12, sc
+    t = grids[gi].tables()
+    say(f"wide rows: {sc[gi]} complex elements per mode, {int(t['branches'])} modes")
+    dev = capi.Grid(t)
+    for precision in (capi.FP32, capi.FP64):
+        rb, vb, cb = run_device(dev, Q, precision, capi.KERNEL_BINNED)
+        rt, vt, ct = run_device(dev, Q, precision, capi.KERNEL_TILE)
+        assert rb == 0 and rt == 0
+        np.testing.assert_array_equal(vb, vt)
+        np.testing.assert_array_equal(cb, ct)
+    m = 4000
+    rc, ov, oc, _, _ = O.ir_interpolate_at(t, Q[:m], nthreads=8)
+    assert rc == 0
+    assert np.max(np.abs(vb[:m] - ov)) <= 1e-12 * np.max(np.abs(ov))
+    assert np.max(np.abs(cb[:m] - oc)) <= 1e-12 * np.max(np.abs(oc))
