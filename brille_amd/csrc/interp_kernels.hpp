@@ -764,12 +764,13 @@ __device__ __forceinline__ void unit_coop(const PointRec<Real>& r, const Chunk<R
 #pragma unroll
     for (int k = 0; k < 2 * CPL; ++k) acc[k] = fma_r(w, rows[x].r[k], acc[k]);
   }
-  // values: pair P >= SC holds values 2 (P - SC) and 2 (P - SC) + 1
+  // values: pair P >= SC holds values 2 (P - SC) and 2 (P - SC) + 1.  The
+  // outputs are written once and not read again: nontemporal stores.
 #pragma unroll
   for (int h = 0; h < CPL; ++h) {
     const int vi = 2 * ((int)q * CPL + h - SC);
-    if (vi >= 0 && vi < SV) ov[vi] = acc[2 * h];
-    if (vi >= 0 && vi + 1 < SV) ov[vi + 1] = acc[2 * h + 1];
+    if (vi >= 0 && vi < SV) __builtin_nontemporal_store(acc[2 * h], ov + vi);
+    if (vi >= 0 && vi + 1 < SV) __builtin_nontemporal_store(acc[2 * h + 1], ov + vi + 1);
   }
   C out[CPL];
   bool put[CPL];
@@ -814,14 +815,15 @@ __device__ __forceinline__ void unit_coop(const PointRec<Real>& r, const Chunk<R
   C* dst = oc + q * CPL;
   if constexpr (CPL == 2) {
     if (put[0] && put[1] && (reinterpret_cast<uintptr_t>(dst) & 15u) == 0) {
-      *reinterpret_cast<typename Cplx<Real>::V16*>(dst) =
-          typename Cplx<Real>::V16{out[0].x, out[0].y, out[1].x, out[1].y};
+      typedef Real real4 __attribute__((ext_vector_type(4)));
+      __builtin_nontemporal_store(real4{out[0].x, out[0].y, out[1].x, out[1].y}, reinterpret_cast<real4*>(dst));
       return;
     }
   }
+  typedef Real real2 __attribute__((ext_vector_type(2)));
 #pragma unroll
   for (int h = 0; h < CPL; ++h)
-    if (put[h]) dst[h] = out[h];
+    if (put[h]) __builtin_nontemporal_store(real2{out[h].x, out[h].y}, reinterpret_cast<real2*>(dst + h));
 }
 
 // Payload layouts with a specialised phase 2 (the general one handles all).
@@ -942,10 +944,12 @@ __device__ inline void unit_wide(const GridConst& g, const DataConst& d, const P
     a1.y = fma_r(w, py, a1.y);
     va = fma_r(w, vv[x], va);
   }
-  if (l < sv) ov[l] = va;
+  // outputs are written once: nontemporal stores
+  typedef Real real2 __attribute__((ext_vector_type(2)));
+  if (l < sv) __builtin_nontemporal_store(va, ov + l);
   if (!gamma) {
-    if (h0) oc[l] = a0;
-    if (h1) oc[l + 64] = a1;
+    if (h0) __builtin_nontemporal_store(real2{a0.x, a0.y}, reinterpret_cast<real2*>(oc + l));
+    if (h1) __builtin_nontemporal_store(real2{a1.x, a1.y}, reinterpret_cast<real2*>(oc + l + 64));
     return;
   }
   // rip_gamma_complex  src/interpolator_gamma.tpp:100-114, element by element
@@ -971,7 +975,8 @@ __device__ inline void unit_wide(const GridConst& g, const DataConst& d, const P
       C t = zero;
 #pragma unroll
       for (int j = 0; j < 3; ++j) t = Elem<C>::add(t, Elem<C>::scale(v[j], wc.R[h][j]));
-      oc[wc.dst[h]] = cmul(wc.ph[h], t);
+      const C o = cmul(wc.ph[h], t);
+      __builtin_nontemporal_store(real2{o.x, o.y}, reinterpret_cast<real2*>(oc + wc.dst[h]));
     }
   }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
