@@ -265,9 +265,13 @@ __global__ __launch_bounds__(PERM_BLOCK) void k_bucket_perm(const uint32_t* node
 // length is read on the device.
 __global__ __launch_bounds__(256) void k_bin_propose(GridConst g, ProposalConst pc, const FoldRec* rec,
                                                      const uint32_t* list, const uint32_t* count, uint32_t* hint) {
+  // one chunk of 16 listed points per wave, four per block; the grid covers
+  // the largest possible list and blocks past the list's end leave at once
+  const uint32_t n = *count;
+  if (blockIdx.x * 64u >= n) return;
   __shared__ float s_feat[4][48];
   __shared__ uint32_t s_t0[4][16], s_nc[4][16], s_out[4][16];
-  // the first two layers' weights, held by the block for all its chunks
+  // the first two layers' weights, from LDS copies
   __shared__ float s_w1[3 * kPropH], s_b1[kPropH], s_b2[kPropH];
   __shared__ __attribute__((aligned(16))) uint16_t s_w2[kPropH * kPropH];
   for (uint32_t i = threadIdx.x; i < 3 * kPropH; i += 256) s_w1[i] = pc.W1[i];
@@ -280,36 +284,31 @@ __global__ __launch_bounds__(256) void k_bin_propose(GridConst g, ProposalConst 
   __syncthreads();
   const PropWeights<LdsF, LdsH> W{(LdsF)s_w1, (LdsF)s_b1, (LdsF)s_b2, (LdsH)s_w2};
   const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
-  const uint32_t n = *count;
-  const uint32_t waves = gridDim.x * 4u;
-  for (uint32_t c0 = (blockIdx.x * 4u + wave) * 16u; c0 < n; c0 += waves * 16u) {
-    const uint32_t k = n - c0 < 16u ? n - c0 : 16u;
-    if (lane < k) {
-      const FoldRec r = rec[list[c0 + lane]];
-      double x[3];
-      mat_vec(x, g.outer_to_xyz, r.q);
-      float f[3];
-      prop_features(g, x, r.node, f);
-      s_feat[wave][3 * lane] = f[0];
-      s_feat[wave][3 * lane + 1] = f[1];
-      s_feat[wave][3 * lane + 2] = f[2];
-      const uint32_t pi = g.node_index[r.node];
-      const uint32_t a = g.poly_off[pi];
-      s_t0[wave][lane] = a;
-      s_nc[wave][lane] = g.poly_off[pi + 1] - a;
-    }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    propose16(pc, W, s_feat[wave], s_t0[wave], s_nc[wave], k, s_out[wave]);
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    if (lane < k) hint[list[c0 + lane]] = s_out[wave][lane];
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  const uint32_t c0 = (blockIdx.x * 4u + wave) * 16u;
+  if (c0 >= n) return;  // wave-uniform; no block-wide barrier follows
+  const uint32_t k = n - c0 < 16u ? n - c0 : 16u;
+  if (lane < k) {
+    const FoldRec r = rec[list[c0 + lane]];
+    double x[3];
+    mat_vec(x, g.outer_to_xyz, r.q);
+    float f[3];
+    prop_features(g, x, r.node, f);
+    s_feat[wave][3 * lane] = f[0];
+    s_feat[wave][3 * lane + 1] = f[1];
+    s_feat[wave][3 * lane + 2] = f[2];
+    const uint32_t pi = g.node_index[r.node];
+    const uint32_t a = g.poly_off[pi];
+    s_t0[wave][lane] = a;
+    s_nc[wave][lane] = g.poly_off[pi + 1] - a;
   }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  propose16_inline(pc, W, s_feat[wave], s_t0[wave], s_nc[wave], k, s_out[wave]);
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  if (lane < k) hint[list[c0 + lane]] = s_out[wave][lane];
 }
 
 // XCD-aware tile order: the hardware hands consecutive blocks to the eight
@@ -738,7 +737,7 @@ int launch_binned(brille_grid* G, const LaunchArgs& a, int precision, hipStream_
     hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(BIN_BLOCK), 0, s, part, np);
     hipLaunchKernelGGL(k_scan_apply, dim3(np), dim3(BIN_BLOCK), 0, s, hist, M, part);
     if (prop)
-      hipLaunchKernelGGL(k_bin_propose, dim3(G->n_cu ? 4 * G->n_cu : 1024), dim3(256), 0, s, G->g, a.prop, rec, plist,
+      hipLaunchKernelGGL(k_bin_propose, dim3((m + 63) / 64), dim3(256), 0, s, G->g, a.prop, rec, plist,
                          pcount, phint);
     hipLaunchKernelGGL(k_bin_scatter, dim3(nblk), dim3(BIN_BLOCK), 2 * lds1, s, rec, m, hist, nblk, nb,
                        G->g.n_nodes, srt, snode, phint);

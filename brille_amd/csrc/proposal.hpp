@@ -56,11 +56,9 @@ __device__ __forceinline__ bf16x8_t load_bf16x8(LdsH p) {
 // Proposals for the n <= 16 points of a chunk (feat: 3 floats per point,
 // t0/nc: first candidate and candidate count).  Called by a whole wave with
 // wave-uniform arguments; out[i] receives the first candidate of largest logit.
-// Out of line: inlined into a loop over chunks, its per-lane layer-1 weights
-// would be hoisted and held across iterations (over a hundred registers).
 template <typename F, typename H>
-__device__ __noinline__ void propose16(const ProposalConst& pc, PropWeights<F, H> W, const float* feat,
-                                       const uint32_t* t0, const uint32_t* nc, uint32_t n, uint32_t* out) {
+__device__ __forceinline__ void propose16_inline(const ProposalConst& pc, PropWeights<F, H> W, const float* feat,
+                                                 const uint32_t* t0, const uint32_t* nc, uint32_t n, uint32_t* out) {
   const uint32_t l = threadIdx.x & 63u, m = l & 15u, grp = l >> 4;
   float x0 = 0.f, x1 = 0.f, x2 = 0.f;
   if (m < n) {
@@ -147,6 +145,15 @@ __device__ __noinline__ void propose16(const ProposalConst& pc, PropWeights<F, H
       if (pi < n) out[pi] = bestc[r];
     }
   }
+}
+
+// Out-of-line form for callers that loop over chunks (inlined into such a
+// loop, the per-lane layer-1 weights would be hoisted and held across
+// iterations, over a hundred registers).
+template <typename F, typename H>
+__device__ __noinline__ void propose16(const ProposalConst& pc, PropWeights<F, H> W, const float* feat,
+                                       const uint32_t* t0, const uint32_t* nc, uint32_t n, uint32_t* out) {
+  propose16_inline(pc, W, feat, t0, nc, n, out);
 }
 
 }  // namespace brille_amd
