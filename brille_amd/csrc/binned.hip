@@ -591,10 +591,11 @@ static int build_wide_factors(brille_grid* G, hipStream_t s, typename Cplx<Real>
   return BRILLE_OK;
 }
 
-// Builds the table of the given precision once per payload (freed with it),
-// eight rows per cube: payloads whose sums are not the one-pass kind, grids
-// without cubes and tables beyond kCubeRowsMax go without, and their sums
-// phase every row on the fly.
+// Builds the payload's derived table of the given precision once (freed with
+// the payload): the antiphased rows (8 per cube, 4 per tetrahedron and mode)
+// for the one-pass layouts, the antiphase factors alone for wide rows.  The
+// general layout, grids without cells and tables beyond kCubeRowsMax go
+// without, and their sums compute every antiphase on the fly.
 constexpr uint64_t kCubeRowsMax = (4ull << 30) - 1;  // the sums address the table with 32-bit offsets
 int ensure_cube_rows(brille_grid* G, int precision, hipStream_t s) {
   DataConst& d = G->d;
