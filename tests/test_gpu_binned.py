@@ -126,3 +126,36 @@ def test_wide_rows_binned_equals_tile_and_oracle():
     assert rc == 0
     assert np.max(np.abs(vb[:m] - ov)) <= 1e-12 * np.max(np.abs(ov))
     assert np.max(np.abs(cb[:m] - oc)) <= 1e-12 * np.max(np.abs(oc))
+
+
+def test_binned_on_vertices_faces_and_edges():
+    """Points on trellis vertices, cube edges and faces (weights exactly 0, so
+    the slot-0 vertex is often dropped and the row table cannot serve the
+    point) mixed with random points: binned equals the fused kernel bit for
+    bit in FP32 and FP64, and FP64 matches the oracle."""
+    g = S.grid(4, volume_fraction=1e-4)
+    t = g.tables()
+    dev = capi.Grid(t)
+    verts = np.asarray(t["vertices"])
+    cubes = np.asarray(t["cube_vertices"]).reshape(-1, 8)
+    r = np.random.default_rng(31)
+    pick = cubes[r.integers(0, len(cubes), 6000)]
+    a, b = pick[:, 0], pick[:, r.integers(1, 8)]
+    on_vertex = verts[a]
+    on_segment = 0.5 * (verts[a] + verts[b])  # an edge, a face diagonal or the body diagonal
+    B = np.asarray(t["outer_to_xyz"]).reshape(3, 3)
+    special = np.linalg.solve(B, np.vstack([on_vertex, on_segment]).T).T
+    Q = np.vstack([special, S.q_points(4, 50_000, stream=32)])
+    Q = Q[r.permutation(len(Q))]
+    say(f"binned on vertices/edges/faces: {len(special)} special of {len(Q)}")
+    for precision in (capi.FP32, capi.FP64):
+        rb, vb, cb = run_device(dev, Q, precision, capi.KERNEL_BINNED)
+        rt, vt, ct = run_device(dev, Q, precision, capi.KERNEL_TILE)
+        assert rb == 0 and rt == 0
+        np.testing.assert_array_equal(vb, vt)
+        np.testing.assert_array_equal(cb, ct)
+    m = 15_000
+    rc, ov, oc, _, _ = O.ir_interpolate_at(t, Q[:m], nthreads=8)
+    assert rc == 0
+    np.testing.assert_allclose(vb[:m], ov, rtol=1e-12, atol=1e-12 * np.abs(ov).max())
+    np.testing.assert_allclose(cb[:m], oc, rtol=1e-12, atol=1e-12)
