@@ -142,6 +142,7 @@ def main():
         learned["eval_hit_rate"] = hits.value / max(1, tried.value)
     results["binned fp32"] = timed(capi.RunOpts(0, capi.FP32, 0, capi.MEM_DEVICE, capi.KERNEL_BINNED, 0), "binned fp32")
     results["tile fp32"] = timed(capi.RunOpts(0, capi.FP32, 0, capi.MEM_DEVICE, capi.KERNEL_TILE, 0), "tile fp32")
+    results["binned fp64"] = timed(capi.RunOpts(0, capi.FP64, 0, capi.MEM_DEVICE, capi.KERNEL_BINNED, 0), "binned fp64")
     plain_res = None
     if not args.no_plain:
         plain_res = timed(capi.RunOpts(0, capi.FP64, 0, capi.MEM_DEVICE, capi.KERNEL_PLAIN, 0), "plain fp64")
