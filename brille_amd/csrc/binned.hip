@@ -34,7 +34,11 @@ struct FoldRec {
   uint32_t node;
   uint32_t meta;  // rot | invrot << 8 | status << 16 | kMetaPoly
 };
-constexpr uint32_t kMetaPoly = 1u << 24;  // the point folded into a polyhedron node
+// the point is on the proposal list: it folded into a polyhedron node of at
+// least kPropMinCand tetrahedra (in smaller nodes the exact scan costs no
+// more than verifying a proposal, and the network is not run)
+constexpr uint32_t kMetaPoly = 1u << 24;
+constexpr uint32_t kPropMinCand = 8;
 __device__ __forceinline__ uint32_t meta_status(uint32_t meta) { return (meta >> 16) & 255u; }
 struct SortRec {
   double q[3];
@@ -48,9 +52,9 @@ __device__ __forceinline__ uint32_t bucket_of(uint32_t node, uint32_t meta, uint
   return meta_status(meta) != BRILLE_PT_OK ? nb : (uint32_t)(((uint64_t)node * nb) / n_nodes);
 }
 
-// poly_list/poly_count (nullable): the points that folded into polyhedron
-// nodes, for the learned proposal (appended per wave: ballot, prefix count,
-// one atomic)
+// poly_list/poly_count (nullable): the points for the learned proposal, those
+// that folded into polyhedron nodes of at least kPropMinCand tetrahedra
+// (appended per wave: ballot, prefix count, one atomic)
 template <bool Fast>
 __global__ __launch_bounds__(BIN_BLOCK) void k_bin_fold(GridConst g, const double* Q, uint32_t n, int fold,
                                                         FoldRec* __restrict__ rec, uint32_t* __restrict__ hist,
@@ -75,6 +79,10 @@ __global__ __launch_bounds__(BIN_BLOCK) void k_bin_fold(GridConst g, const doubl
       fold_node<Fast>(g, Qp, fold, L);
       if (L.status != BRILLE_PT_OK) report_point(err, base + p, L.status);
       poly = L.status == BRILLE_PT_OK && g.node_type[L.node] == BRILLE_NODE_POLY;
+      if (poly && poly_list) {
+        const uint32_t pi = g.node_index[L.node];
+        poly = g.poly_off[pi + 1] - g.poly_off[pi] >= kPropMinCand;
+      }
       FoldRec r;
       r.q[0] = L.q[0]; r.q[1] = L.q[1]; r.q[2] = L.q[2];
       r.node = L.node;
