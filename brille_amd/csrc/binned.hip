@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <vector>
 
 #include "grid_state.hpp"
 
@@ -55,8 +56,10 @@ __device__ __forceinline__ uint32_t bucket_of(uint32_t node, uint32_t meta, uint
 // poly_list/poly_count (nullable): the points for the learned proposal, those
 // that folded into polyhedron nodes of at least kPropMinCand tetrahedra
 // (appended per wave: ballot, prefix count, one atomic)
+// Six waves per SIMD (a few spilled registers) hide more of the FP64 latency
+// than the four that 124 VGPRs allow: 9 % less time.
 template <bool Fast>
-__global__ __launch_bounds__(BIN_BLOCK) void k_bin_fold(GridConst g, const double* Q, uint32_t n, int fold,
+__global__ __launch_bounds__(BIN_BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void k_bin_fold(GridConst g, const double* Q, uint32_t n, int fold,
                                                         FoldRec* __restrict__ rec, uint32_t* __restrict__ hist,
                                                         uint32_t nblk, uint32_t nb,
                                                         ErrRec* __restrict__ err, uint64_t base,
@@ -685,22 +688,73 @@ bool binned_applies(const brille_grid* G, const LaunchArgs& a, int kernel) {
   return a.n >= (1ull << 18) && payload > (8ull << 20);
 }
 
-void free_binned(brille_grid* G) {
+__global__ void k_zero_words(uint32_t* p, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+void zero_words(void* p, uint32_t n, hipStream_t s) {
+  if (n) hipLaunchKernelGGL(k_zero_words, dim3((n + 255) / 256), dim3(256), 0, s, static_cast<uint32_t*>(p), n);
+}
+
+static void free_work_space(brille_grid* G) {
   if (G->bin_ws) (void)hipFree(G->bin_ws);
   G->bin_ws = nullptr;
   G->bin_ws_bytes = 0;
 }
 
-// stage_ms (nullable): adds the milliseconds of fold, bucket (scan + scatter)
-// and sums over all passes; timing synchronises after every pass
+void free_binned(brille_grid* G) {
+  free_work_space(G);
+  for (auto& e : G->bin_ev) {
+    if (e) (void)hipEventDestroy(e);
+    e = nullptr;
+  }
+  if (G->bin_side) (void)hipStreamDestroy(G->bin_side);
+  G->bin_side = nullptr;
+}
+
+// the side stream and events of the two-stage pipeline, made on first use
+// (default priority) and kept with the grid
+static int ensure_side_stream(brille_grid* G) {
+  if (G->bin_side) return BRILLE_OK;
+  for (auto& e : G->bin_ev)
+    if (!e && hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
+      return fail(BRILLE_ERR_DEVICE, "hipEventCreate failed");
+  if (hipStreamCreateWithFlags(&G->bin_side, hipStreamNonBlocking) != hipSuccess) {
+    G->bin_side = nullptr;
+    return fail(BRILLE_ERR_DEVICE, "hipStreamCreate failed for the binned pipeline");
+  }
+  return BRILLE_OK;
+}
+
+// One half of the work space: the buffers of one pass.
+struct PassBuffers {
+  FoldRec* rec;
+  SortRec* srt;
+  uint32_t *hist, *part, *plist, *phint, *pcount, *perm, *snode;
+};
+
+// A call's passes (BIN_CHUNK points each) run as a two-stage pipeline when
+// there are several: the preparation of pass k + 1 (fold, scans, proposal,
+// scatter, node order) on the grid's side stream while the sums of pass k run
+// on the call's stream, each pass with its own half of the work space.  The
+// fold is bound by FP64 issue and the sums by their scattered stores, and one
+// fold wave per SIMD fits beside four sums waves, so the two overlap.  The
+// call stream waits for each preparation before its sums, the side stream for
+// the sums that last used a half, and the side stream starts after what the
+// call stream held before the call: the call is ordered on `s` like a single
+// kernel.
+//
+// stage_ms (nullable): adds the milliseconds of fold, bucket (scans, proposal,
+// scatter, node order) and sums over all passes, each from events on the
+// stream that ran it (with several passes the stages overlap, so they add up
+// to more than the call)
 int launch_binned(brille_grid* G, const LaunchArgs& a, int precision, hipStream_t s, float* stage_ms) {
   if (int rc = ensure_cube_rows(G, precision, s); rc != BRILLE_OK) return rc;
-  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-  if (stage_ms)
-    for (auto& e : ev)
-      if (hipEventCreate(&e) != hipSuccess) return fail(BRILLE_ERR_DEVICE, "hipEventCreate failed");
-  hipEvent_t* evp = stage_ms ? ev : nullptr;
   const uint64_t chunk = std::min<uint64_t>(a.n, BIN_CHUNK);
+  if (chunk == 0) return BRILLE_OK;
+  const uint64_t passes = (a.n + chunk - 1) / chunk;
+  const int halves = passes > 1 ? 2 : 1;
   const uint32_t nb = bucket_count(G);
   const uint32_t nblk_max = (uint32_t)((chunk + BIN_SPAN - 1) / BIN_SPAN);
   const uint64_t m_max = (uint64_t)(nb + 1) * nblk_max;
@@ -709,65 +763,109 @@ int launch_binned(brille_grid* G, const LaunchArgs& a, int precision, hipStream_
   const uint64_t o_rec = 0, o_srt = al(chunk * sizeof(FoldRec)), o_hist = o_srt + al(chunk * sizeof(SortRec)),
                  o_part = o_hist + al(m_max * 4), o_list = o_part + al(np_max * 4), o_hint = o_list + al(chunk * 4),
                  o_perm = o_hint + al(chunk * 4), o_node = o_perm + al(chunk * 4), o_cnt = o_node + al(chunk * 4),
-                 need = o_cnt + 256;
+                 half = o_cnt + 256, need = half * (uint64_t)halves;
   if (np_max > 256ull * 1024) return fail(BRILLE_ERR_INVALID_ARGUMENT, "binned pass too large");
   if (G->bin_ws_bytes < need) {
-    free_binned(G);
+    free_work_space(G);
     if (hipMalloc(&G->bin_ws, need) != hipSuccess) return fail(BRILLE_ERR_DEVICE, "hipMalloc failed for the binned work space");
     G->bin_ws_bytes = need;
   }
-  char* ws = static_cast<char*>(G->bin_ws);
-  FoldRec* rec = reinterpret_cast<FoldRec*>(ws + o_rec);
-  SortRec* srt = reinterpret_cast<SortRec*>(ws + o_srt);
-  uint32_t* hist = reinterpret_cast<uint32_t*>(ws + o_hist);
-  uint32_t* part = reinterpret_cast<uint32_t*>(ws + o_part);
+  if (halves > 1)
+    if (int rc = ensure_side_stream(G); rc != BRILLE_OK) return rc;
   const bool prop = a.use_proposal != 0;
-  uint32_t* plist = prop ? reinterpret_cast<uint32_t*>(ws + o_list) : nullptr;
-  uint32_t* phint = prop ? reinterpret_cast<uint32_t*>(ws + o_hint) : nullptr;
-  uint32_t* pcount = reinterpret_cast<uint32_t*>(ws + o_cnt);
-  uint32_t* perm = reinterpret_cast<uint32_t*>(ws + o_perm);
-  uint32_t* snode = reinterpret_cast<uint32_t*>(ws + o_node);
+  PassBuffers buf[2];
+  for (int h = 0; h < halves; ++h) {
+    char* ws = static_cast<char*>(G->bin_ws) + h * half;
+    PassBuffers& b = buf[h];
+    b.rec = reinterpret_cast<FoldRec*>(ws + o_rec);
+    b.srt = reinterpret_cast<SortRec*>(ws + o_srt);
+    b.hist = reinterpret_cast<uint32_t*>(ws + o_hist);
+    b.part = reinterpret_cast<uint32_t*>(ws + o_part);
+    b.plist = prop ? reinterpret_cast<uint32_t*>(ws + o_list) : nullptr;
+    b.phint = prop ? reinterpret_cast<uint32_t*>(ws + o_hint) : nullptr;
+    b.pcount = reinterpret_cast<uint32_t*>(ws + o_cnt);
+    b.perm = reinterpret_cast<uint32_t*>(ws + o_perm);
+    b.snode = reinterpret_cast<uint32_t*>(ws + o_node);
+  }
+  // timing: five events per pass (fold start, fold end, bucket end on the
+  // preparing stream; sums start and end on the call stream)
+  std::vector<hipEvent_t> tev;
+  auto destroy_tev = [&tev]() {
+    for (auto e : tev)
+      if (e) (void)hipEventDestroy(e);
+  };
+  if (stage_ms) {
+    tev.assign(passes * 5, nullptr);
+    for (auto& e : tev)
+      if (hipEventCreate(&e) != hipSuccess) {
+        destroy_tev();
+        return fail(BRILLE_ERR_DEVICE, "hipEventCreate failed");
+      }
+  }
+  auto mark = [&](uint64_t k, int j, hipStream_t st) {
+    if (stage_ms) (void)hipEventRecord(tev[k * 5 + j], st);
+  };
   const SumsKernel sums = precision == BRILLE_FP32 ? pick_sums<float>(G->d, prop) : pick_sums<double>(G->d, prop);
   const size_t rs = precision == BRILLE_FP32 ? 4 : 8;
   const size_t vb = (size_t)G->d.B * G->d.sv * rs, cb = (size_t)G->d.B * G->d.sc * 2 * rs;
-  for (uint64_t c0 = 0; c0 < a.n; c0 += chunk) {
+  const size_t lds1 = (size_t)(nb + 1) * 4;
+  hipStream_t side = halves > 1 ? G->bin_side : s;
+  enum { EV_IN = 0, EV_PREP = 1, EV_DONE = 3 };  // G->bin_ev: call start, prepared[2], summed[2]
+  if (halves > 1) {
+    (void)hipEventRecord(G->bin_ev[EV_IN], s);
+    (void)hipStreamWaitEvent(side, G->bin_ev[EV_IN], 0);
+  }
+  for (uint64_t k = 0; k < passes; ++k) {
+    const uint64_t c0 = k * chunk;
+    const int h = (int)(k & 1u) % halves;
+    const PassBuffers& b = buf[h];
     const uint32_t m = (uint32_t)std::min<uint64_t>(chunk, a.n - c0);
     const uint32_t nblk = (m + BIN_SPAN - 1) / BIN_SPAN;
     const uint64_t M = (uint64_t)(nb + 1) * nblk;
     const uint32_t np = (uint32_t)((M + SCAN_SEG - 1) / SCAN_SEG);
-    const size_t lds1 = (size_t)(nb + 1) * 4;
-    if (prop && hipMemsetAsync(pcount, 0, sizeof(uint32_t), s) != hipSuccess)
-      return fail(BRILLE_ERR_DEVICE, "hipMemsetAsync failed");
-    if (evp) (void)hipEventRecord(evp[0], s);
-    hipLaunchKernelGGL(k_bin_fold<true>, dim3(nblk), dim3(BIN_BLOCK), lds1, s, G->g, a.Q + 3 * c0, m, a.fold, rec,
-                       hist, nblk, nb, a.err, a.base + c0, plist, pcount);
-    if (evp) (void)hipEventRecord(evp[1], s);
-    hipLaunchKernelGGL(k_scan_reduce, dim3(np), dim3(BIN_BLOCK), 0, s, hist, M, part);
-    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(BIN_BLOCK), 0, s, part, np);
-    hipLaunchKernelGGL(k_scan_apply, dim3(np), dim3(BIN_BLOCK), 0, s, hist, M, part);
+    // preparation of pass k on the side stream, once the sums of pass k - 2
+    // no longer read this half
+    if (halves > 1 && k >= 2) (void)hipStreamWaitEvent(side, G->bin_ev[EV_DONE + h], 0);
+    if (prop) zero_words(b.pcount, 1, side);
+    mark(k, 0, side);
+    hipLaunchKernelGGL(k_bin_fold<true>, dim3(nblk), dim3(BIN_BLOCK), lds1, side, G->g, a.Q + 3 * c0, m, a.fold,
+                       b.rec, b.hist, nblk, nb, a.err, a.base + c0, b.plist, b.pcount);
+    mark(k, 1, side);
+    hipLaunchKernelGGL(k_scan_reduce, dim3(np), dim3(BIN_BLOCK), 0, side, b.hist, M, b.part);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(BIN_BLOCK), 0, side, b.part, np);
+    hipLaunchKernelGGL(k_scan_apply, dim3(np), dim3(BIN_BLOCK), 0, side, b.hist, M, b.part);
     if (prop)
-      hipLaunchKernelGGL(k_bin_propose, dim3((m + 63) / 64), dim3(256), 0, s, G->g, a.prop, rec, plist,
-                         pcount, phint);
-    hipLaunchKernelGGL(k_bin_scatter, dim3(nblk), dim3(BIN_BLOCK), 2 * lds1, s, rec, m, hist, nblk, nb,
-                       G->g.n_nodes, srt, snode, phint);
-    hipLaunchKernelGGL(k_bucket_perm, dim3(nb + 1), dim3(PERM_BLOCK), 0, s, snode, m, hist, nblk, nb, G->g.n_nodes,
-                       perm);
-    if (evp) (void)hipEventRecord(evp[2], s);
-    hipLaunchKernelGGL(sums, dim3((m + TILE - 1) / TILE), dim3(TILE), 0, s, G->g, G->d, a.prop, srt, perm, m,
+      hipLaunchKernelGGL(k_bin_propose, dim3((m + 63) / 64), dim3(256), 0, side, G->g, a.prop, b.rec, b.plist,
+                         b.pcount, b.phint);
+    hipLaunchKernelGGL(k_bin_scatter, dim3(nblk), dim3(BIN_BLOCK), 2 * lds1, side, b.rec, m, b.hist, nblk, nb,
+                       G->g.n_nodes, b.srt, b.snode, b.phint);
+    hipLaunchKernelGGL(k_bucket_perm, dim3(nb + 1), dim3(PERM_BLOCK), 0, side, b.snode, m, b.hist, nblk, nb,
+                       G->g.n_nodes, b.perm);
+    mark(k, 2, side);
+    if (halves > 1) {
+      (void)hipEventRecord(G->bin_ev[EV_PREP + h], side);
+      (void)hipStreamWaitEvent(s, G->bin_ev[EV_PREP + h], 0);
+    }
+    // sums of pass k on the call stream
+    mark(k, 3, s);
+    hipLaunchKernelGGL(sums, dim3((m + TILE - 1) / TILE), dim3(TILE), 0, s, G->g, G->d, a.prop, b.srt, b.perm, m,
                        static_cast<char*>(a.vals) + c0 * vb, static_cast<char*>(a.vecs) + c0 * cb,
                        offset_aux(a.aux, c0), a.err, a.base + c0);
-    if (evp) {
-      (void)hipEventRecord(evp[3], s);
-      (void)hipEventSynchronize(evp[3]);
-      for (int k = 0; k < 3; ++k) {
+    mark(k, 4, s);
+    if (halves > 1) (void)hipEventRecord(G->bin_ev[EV_DONE + h], s);
+  }
+  if (stage_ms) {
+    (void)hipEventSynchronize(tev[(passes - 1) * 5 + 4]);
+    for (uint64_t k = 0; k < passes; ++k) {
+      const int from[3] = {0, 1, 3}, to[3] = {1, 2, 4};
+      for (int j = 0; j < 3; ++j) {
         float ms = 0;
-        (void)hipEventElapsedTime(&ms, evp[k], evp[k + 1]);
-        stage_ms[k] += ms;
+        (void)hipEventElapsedTime(&ms, tev[k * 5 + from[j]], tev[k * 5 + to[j]]);
+        stage_ms[j] += ms;
       }
     }
+    destroy_tev();
   }
-  if (evp)
-    for (auto& e : ev) (void)hipEventDestroy(e);
   if (hipGetLastError() != hipSuccess) return fail(BRILLE_ERR_DEVICE, "binned launch failed");
   return BRILLE_OK;
 }

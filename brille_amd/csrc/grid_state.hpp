@@ -24,6 +24,10 @@ struct brille_grid {
   // work space of the binned pipeline (grown on demand, kept)
   void* bin_ws = nullptr;
   size_t bin_ws_bytes = 0;
+  // side stream of the binned pipeline (preparation of the next pass) and its
+  // events: call start, prepared[2], summed[2]
+  hipStream_t bin_side = nullptr;
+  hipEvent_t bin_ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   uint32_t n_cu = 0;  // compute units of the device (grid sizes of grid-stride kernels)
   // learned tetrahedron proposal: FP32 master parameters (flat: W1 b1 W2 b2
   // W3 b3), Adam moments and gradients; bf16 inference copies in `prop`
@@ -54,6 +58,9 @@ bool wide_rows(const DataConst& d);
 bool binned_applies(const brille_grid* G, const LaunchArgs& a, int kernel);
 int launch_binned(brille_grid* G, const LaunchArgs& a, int precision, hipStream_t s, float* stage_ms);
 void free_binned(brille_grid* G);
+// clears n 32-bit words on stream s with a kernel (no memset node, so that a
+// caller may capture the call in a graph)
+void zero_words(void* p, uint32_t n, hipStream_t s);
 // the cube row table of the bucketed sums, built on first use (data_allocs)
 int ensure_cube_rows(brille_grid* G, int precision, hipStream_t s);
 // the first failing point of a call as the reference's error text

@@ -312,8 +312,11 @@ int brille_proposal_predict(brille_grid* grid, const float* x, const int32_t* in
                             int32_t* tet_out);
 
 /* ---- timing and device plumbing (used by the benchmark and tests) ---- */
-/* run the hot path `iters` times on `stream` and report per-kernel mean launch
- * times (ms) measured with HIP events recorded on that stream */
+/* run the hot path `iters` times on `stream` and report mean times (ms)
+ * measured with HIP events: total_ms is one whole call (events on `stream`
+ * around all iterations); ms[] are the kernels (fused path) or the stages of
+ * the binned pipeline, timed on the stream each ran on (with several passes
+ * the binned stages overlap and add up to more than total_ms) */
 typedef struct brille_kernel_times {
   uint32_t n_kernels;
   char names[4][32];

@@ -91,18 +91,27 @@ def test_binned_errors_match_tile():
 
 
 def test_binned_several_passes():
-    """More points than one pass holds (2^24): the passes tile the input exactly."""
-    n = (1 << 24) + 4099
+    """More points than one pass holds (2^24): three passes, the third reusing
+    the work space half of the first while the second is summed; the passes
+    tile the input exactly."""
+    n = (2 << 24) + 4099
     say(f"binned several passes n={n}")
     g = S.grid(0)
     dev = capi.Grid(g.tables())
     Q = S.q_points(0, n, stream=14)
-    Q[(1 << 24) + 17] = [np.nan, 0.0, 0.0]
+    bad = (2 << 24) + 17
+    Q[bad] = [np.nan, 0.0, 0.0]
     rb, vb, cb = run_device(dev, Q, capi.FP32, capi.KERNEL_BINNED)
     rt, vt, ct = run_device(dev, Q, capi.FP32, capi.KERNEL_TILE)
     assert rb == rt != 0
+    assert np.all(vb[bad] == 0) and np.all(cb[bad] == 0)
     np.testing.assert_array_equal(vb, vt)
     np.testing.assert_array_equal(cb, ct)
+    # the same call twice in a row on one stream gives the same outputs
+    rb2, vb2, cb2 = run_device(dev, Q, capi.FP32, capi.KERNEL_BINNED)
+    assert rb2 == rb
+    np.testing.assert_array_equal(vb2, vb)
+    np.testing.assert_array_equal(cb2, cb)
 
 
 def test_wide_rows_binned_equals_tile_and_oracle():

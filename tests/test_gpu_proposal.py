@@ -158,3 +158,51 @@ def test_trained_proposal_keeps_the_reference_choice():
     rc, ov, oc, oaux, _ = O.ir_interpolate_at(t, Q[:m], nthreads=8)
     assert rc == 0
     np.testing.assert_array_equal(a1["tet"][:m], oaux["tet"])
+
+
+def _same_device_bytes(a, b, step=1 << 28):
+    """Byte-wise equality of two device arrays, copied to the host in pieces."""
+    lib = capi.lib()
+    assert a.nbytes == b.nbytes
+    ha = np.empty(step, np.uint8)
+    hb = np.empty(step, np.uint8)
+    for off in range(0, a.nbytes, step):
+        k = min(step, a.nbytes - off)
+        capi.check(lib.brille_memcpy_d2h(ha.ctypes.data_as(C.c_void_p), C.c_void_p(a.ptr.value + off), k))
+        capi.check(lib.brille_memcpy_d2h(hb.ctypes.data_as(C.c_void_p), C.c_void_p(b.ptr.value + off), k))
+        if not np.array_equal(ha[:k], hb[:k]):
+            return False
+    return True
+
+
+def test_learned_proposal_over_several_passes():
+    """Three passes of 2^24 points with the proposal on: each pass keeps its own
+    list, hints and counter while the next is prepared; the outputs equal those
+    of the binned pipeline without the proposal, byte for byte."""
+    g = S.grid(4, volume_fraction=1e-4)
+    t = g.tables()
+    dev = capi.Grid(t)
+    lib = capi.lib()
+    opts = capi.TrainOpts(400_000, 9, 2, 8192, 2e-3)
+    rep = capi.TrainReport()
+    capi.check(lib.brille_proposal_train(dev.h, C.byref(opts), C.byref(rep)))
+    n = (2 << 24) + 4099
+    say(f"learned proposal over several passes: n={n}, held-out hit rate {rep.train_hit_rate:.3f}")
+    Q = S.q_points(4, n, stream=23)
+    dQ = capi.DeviceArray.from_host(Q)
+    del Q
+    vs, cs = dev.out_shapes(n)
+    outs = []
+    for use in (True, False):
+        dv = capi.DeviceArray(vs, np.float32)
+        dc = capi.DeviceArray(cs, np.complex64)
+        o = capi.RunOpts(0, capi.FP32, int(use), capi.MEM_DEVICE, capi.KERNEL_BINNED, 1)
+        capi.check(lib.brille_ir_interpolate_at(dev.h, dQ.ptr, n, C.byref(o), dv.ptr, dc.ptr, None, None))
+        if use:
+            tried, hits = C.c_uint64(), C.c_uint64()
+            capi.check(lib.brille_proposal_stats(dev.h, C.byref(tried), C.byref(hits)))
+            say(f"{tried.value} proposals, {hits.value} taken")
+            assert tried.value > 0 and hits.value > 0.5 * tried.value
+        outs.append((dv, dc))
+    assert _same_device_bytes(outs[0][0], outs[1][0])
+    assert _same_device_bytes(outs[0][1], outs[1][1])
