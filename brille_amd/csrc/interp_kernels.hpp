@@ -1148,11 +1148,39 @@ __device__ __forceinline__ uint32_t table_row(const DataConst& d, const Located&
   return kNoCube;
 }
 
-// a located point's phase-2 record (plus phases, and q for the general layout)
+// the Gamma frame of point-group operation invrot (it depends on nothing else)
+template <typename Real, int SC>
+__device__ __forceinline__ void fill_frame(const GridConst& g, const DataConst& d, uint32_t invrot,
+                                           GammaFrame<Real>& f) {
+  if (d.vec_lu == BRILLE_LU_ANGSTROM) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) f.R[k] = (Real)d.cart_rot[9 * (uint64_t)invrot + k];
+  } else {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) f.R[k] = (Real)g.rot[9 * (uint64_t)invrot + k];
+  }
+  uint8_t to[4] = {255, 255, 255, 255}, from[4] = {255, 255, 255, 255};
+#pragma unroll
+  for (int a = 0; a < SC / 3; ++a) {
+    const uint32_t l = d.F0[a * d.gamma_ops + invrot];
+    to[a] = (uint8_t)l;
+#pragma unroll
+    for (int o = 0; o < SC / 3; ++o)
+      if ((uint32_t)o == l) from[o] = (uint8_t)a;
+  }
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    f.to[a] = to[a];
+    f.from[a] = from[a];
+  }
+}
+
+// a located point's phase-2 record (plus phases, and q for the general
+// layout; frame = false: the Gamma frame comes from a per-operation table)
 template <typename Real, int L, int SC>
 __device__ __forceinline__ void store_record(const GridConst& g, const DataConst& d, const Located& P,
                                              PointRec<Real>& r, typename Cplx<Real>::T* s_ph, double* s_q,
-                                             uint32_t slot, bool ph_in_lds, uint32_t n_atoms) {
+                                             uint32_t slot, bool ph_in_lds, uint32_t n_atoms, bool frame = true) {
   constexpr int NPH = TileShape<L, SC>::NPH;
 #pragma unroll
   for (int s = 0; s < 8; ++s) {
@@ -1167,28 +1195,7 @@ __device__ __forceinline__ void store_record(const GridConst& g, const DataConst
   if constexpr (L == LAYOUT_GAMMA) {
 #pragma unroll
     for (int k = 0; k < NPH; ++k) s_ph[NPH * slot + k] = gamma_phase<Real>(d, P.q, P.invrot, k);
-    GammaFrame<Real>& f = reinterpret_cast<GammaFrame<Real>*>(s_q)[slot];
-    if (d.vec_lu == BRILLE_LU_ANGSTROM) {
-#pragma unroll
-      for (int k = 0; k < 9; ++k) f.R[k] = (Real)d.cart_rot[9 * (uint64_t)P.invrot + k];
-    } else {
-#pragma unroll
-      for (int k = 0; k < 9; ++k) f.R[k] = (Real)g.rot[9 * (uint64_t)P.invrot + k];
-    }
-    uint8_t to[4] = {255, 255, 255, 255}, from[4] = {255, 255, 255, 255};
-#pragma unroll
-    for (int a = 0; a < SC / 3; ++a) {
-      const uint32_t l = d.F0[a * d.gamma_ops + P.invrot];
-      to[a] = (uint8_t)l;
-#pragma unroll
-      for (int o = 0; o < SC / 3; ++o)
-        if ((uint32_t)o == l) from[o] = (uint8_t)a;
-    }
-#pragma unroll
-    for (int a = 0; a < 4; ++a) {
-      f.to[a] = to[a];
-      f.from[a] = from[a];
-    }
+    if (frame) fill_frame<Real, SC>(g, d, P.invrot, reinterpret_cast<GammaFrame<Real>*>(s_q)[slot]);
   } else if constexpr (L == LAYOUT_GENERIC) {
     s_q[3 * slot] = P.q[0]; s_q[3 * slot + 1] = P.q[1]; s_q[3 * slot + 2] = P.q[2];
     if (ph_in_lds)
@@ -1208,7 +1215,8 @@ __device__ __forceinline__ void tile_sums(const GridConst& g, const DataConst& d
                                           const typename Cplx<Real>::T* s_ph, const double* s_q,
                                           const uint32_t* out_row, uint64_t row0, uint32_t npts, bool ph_in_lds,
                                           void* vals_, void* vecs_, typename Cplx<Real>::T* s_wide = nullptr,
-                                          bool table = false, uint32_t n_coop = 0) {
+                                          bool table = false, uint32_t n_coop = 0,
+                                          const GammaFrame<Real>* op_frames = nullptr) {
   using C = typename Cplx<Real>::T;
   constexpr int NPH = TileShape<L, SC>::NPH;
   const uint32_t B = d.B;
@@ -1240,7 +1248,9 @@ __device__ __forceinline__ void tile_sums(const GridConst& g, const DataConst& d
     }
     return;
   }
+  // Gamma frames: per point in s_q, or per point-group operation in op_frames
   const GammaFrame<Real>* frames = reinterpret_cast<const GammaFrame<Real>*>(s_q);
+  auto frame_of = [&](uint32_t pl) { return op_frames ? op_frames + s_rec[pl].invrot : frames + pl; };
   uint32_t u0 = 0;
   constexpr bool kCoop = (L == LAYOUT_GAMMA || L == LAYOUT_SCALAR) && coop_lanes<Real, SC, SV>() > 1;
   if constexpr (kCoop) {
@@ -1269,7 +1279,7 @@ __device__ __forceinline__ void tile_sums(const GridConst& g, const DataConst& d
           }
           const uint64_t row = ((out_row ? (uint64_t)out_row[pl] : row0 + pl)) * B + b;
           unit_coop<Real, SC, SV, L == LAYOUT_GAMMA, (int)G>(r, rows, q, vals + row * d.sv, vecs + row * d.sc,
-                                                             s_ph + NPH * pl, frames + pl);
+                                                             s_ph + NPH * pl, frame_of(pl));
         }
       } else {
         constexpr uint32_t STEP = NG;  // units per block and iteration
@@ -1282,7 +1292,7 @@ __device__ __forceinline__ void tile_sums(const GridConst& g, const DataConst& d
           const uint64_t row = ((out_row ? (uint64_t)out_row[pl] : row0 + pl)) * B + b;
           coop_load<Real>(d, r, (r.trow + b) * rb + q * 16u, slot_bytes, rows);
           unit_coop<Real, SC, SV, L == LAYOUT_GAMMA, (int)G>(r, rows, q, vals + row * d.sv, vecs + row * d.sc,
-                                                             s_ph + NPH * pl, frames + pl);
+                                                             s_ph + NPH * pl, frame_of(pl));
           b += sb;
           pl += sp;
           if (b >= B) {
@@ -1307,7 +1317,7 @@ __device__ __forceinline__ void tile_sums(const GridConst& g, const DataConst& d
     }
     // with the cooperative sums, every table point went there
     if constexpr (L == LAYOUT_GAMMA) {
-      unit_onepass<Real, SC, SV, true, GRP>(d, r, b, ov, oc, s_ph + NPH * pl, frames + pl, table && !kCoop);
+      unit_onepass<Real, SC, SV, true, GRP>(d, r, b, ov, oc, s_ph + NPH * pl, frame_of(pl), table && !kCoop);
     } else if constexpr (L == LAYOUT_SCALAR) {
       unit_onepass<Real, SC, SV, false, GRP>(d, r, b, ov, oc, nullptr, nullptr, table && !kCoop);
     } else {
