@@ -279,8 +279,8 @@ __global__ __launch_bounds__(PERM_BLOCK) void k_bucket_perm(const uint32_t* node
 // length is read on the device.
 __global__ __launch_bounds__(256) void k_bin_propose(GridConst g, ProposalConst pc, const FoldRec* rec,
                                                      const uint32_t* list, const uint32_t* count, uint32_t* hint) {
-  // one chunk of 16 listed points per wave, four per block; the grid covers
-  // the largest possible list and blocks past the list's end leave at once
+  // chunks of 16 listed points per wave, four waves per block, grid-stride
+  // over the list; blocks past the list's end leave before copying weights
   const uint32_t n = *count;
   if (blockIdx.x * 64u >= n) return;
   __shared__ float s_feat[4][48];
@@ -298,8 +298,8 @@ __global__ __launch_bounds__(256) void k_bin_propose(GridConst g, ProposalConst 
   __syncthreads();
   const PropWeights<LdsF, LdsH> W{(LdsF)s_w1, (LdsF)s_b1, (LdsF)s_b2, (LdsH)s_w2};
   const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
-  const uint32_t c0 = (blockIdx.x * 4u + wave) * 16u;
-  if (c0 >= n) return;  // wave-uniform; no block-wide barrier follows
+  // wave-uniform loop; no block-wide barrier follows
+  for (uint32_t c0 = (blockIdx.x * 4u + wave) * 16u; c0 < n; c0 += gridDim.x * 64u) {
   const uint32_t k = n - c0 < 16u ? n - c0 : 16u;
   if (lane < k) {
     const FoldRec r = rec[list[c0 + lane]];
@@ -323,6 +323,11 @@ __global__ __launch_bounds__(256) void k_bin_propose(GridConst g, ProposalConst 
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   if (lane < k) hint[list[c0 + lane]] = s_out[wave][lane];
+  // the next chunk overwrites this wave's LDS slices
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
 }
 
 // XCD-aware tile order: the hardware hands consecutive blocks to the eight
@@ -846,8 +851,8 @@ int launch_binned(brille_grid* G, const LaunchArgs& a, int precision, hipStream_
     hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(BIN_BLOCK), 0, side, b.part, np);
     hipLaunchKernelGGL(k_scan_apply, dim3(np), dim3(BIN_BLOCK), 0, side, b.hist, M, b.part);
     if (prop)
-      hipLaunchKernelGGL(k_bin_propose, dim3((m + 63) / 64), dim3(256), 0, side, G->g, a.prop, b.rec, b.plist,
-                         b.pcount, b.phint);
+      hipLaunchKernelGGL(k_bin_propose, dim3(std::min<uint32_t>((m + 63) / 64, std::max<uint32_t>(G->n_cu, 1) * 4)),
+                         dim3(256), 0, side, G->g, a.prop, b.rec, b.plist, b.pcount, b.phint);
     hipLaunchKernelGGL(k_bin_scatter, dim3(nblk), dim3(BIN_BLOCK), 2 * lds1, side, b.rec, m, b.hist, nblk, nb,
                        G->g.n_nodes, b.srt, b.snode, b.phint);
     hipLaunchKernelGGL(k_bucket_perm, dim3(nb + 1), dim3(PERM_BLOCK), 0, side, b.snode, m, b.hist, nblk, nb,
