@@ -173,6 +173,10 @@ def main():
     # HIP-event time of all its kernels.  The largest stage is the sums kernel.
     achieved = alg / (k_fast * 1e-3) / 1e9
     per_q = measured_traffic(cfg, head_name)
+    # a call runs in passes of 2^24 points (one k_bin_sums dispatch each); with
+    # several passes the next pass is prepared while the sums run, so a
+    # stage's busy time can exceed its share of the call
+    passes = -(-n // (1 << 24))
     sums_ms = head["stages"].get("k_bin_sums")
     roofline = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
                     traffic=(per_q * n if per_q else None), bytes_per_q=bytes_per_q, table_bytes=tb,
@@ -181,6 +185,8 @@ def main():
                     kernel="binned pipeline (k_bin_fold, k_scan_*, k_bin_scatter, k_bucket_perm, "
                            "k_bin_propose when learned, k_bin_sums)",
                     dominant_kernel=dict(name="k_bin_sums<float, Gamma layout>", ms=sums_ms,
+                                         dispatches_per_call=passes,
+                                         avg_dispatch_ms=(sums_ms / passes if sums_ms else None),
                                          share=(sums_ms / k_fast if sums_ms else None),
                                          output_bytes_per_q=B * sv * 4 + B * sc * 8,
                                          output_gbs=(n * (B * sv * 4 + B * sc * 8) / (sums_ms * 1e-3) / 1e9
