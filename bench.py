@@ -55,7 +55,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              "(the path shards by Q with no exchange; see DESIGN.md)", file=sys.stderr)
+              "(the path would shard by Q with no exchange between GPUs)", file=sys.stderr)
         sys.exit(2)
 
     if args.copy_probe_mib:
