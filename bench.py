@@ -5,8 +5,11 @@ Workload (default): the learned-proposal configuration -- hexagonal P6_3/mmc,
 a=b=3.2 A, c=5.2 A, 2 atoms, 6 modes with complex eigenvectors, trellis node
 volume 1e-5 of the irreducible volume, 1e8 Q uniform in [-4,4]^3 rlu,
 inputs and outputs resident in HBM.  One step = one ir_interpolate_at pass over
-all Q.  The headline path is the tuned FP32 kernel (FP64 fold-in, indices and
-weights; FP32/complex64 sums); the plain FP64 kernel is timed beside it.
+all Q.  The headline path is the binned FP32 pipeline with the learned
+tetrahedron proposal (FP64 fold-in, indices and weights; FP32/complex64 sums);
+the binned pipeline without the proposal, the fused tile kernel, the binned
+FP64 path and the plain FP64 kernel are timed beside it, and the oracle on the
+host cores.  `--config 2` times the batch of 64 uneven grids instead.
 
 Prints one JSON line.
 """
