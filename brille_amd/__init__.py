@@ -14,7 +14,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 try:
     from . import _brille  # noqa: F401
 except ImportError as exc:  # pragma: no cover - exercised only on a broken build
-    raise ImportError("brille_amd native extension is not built; run `python -m brille_amd.build` "
+    raise ImportError("brille_amd native extension is not built; run `python brille_amd/build.py` "
                       f"({exc})") from exc
 
 from ._brille import (  # noqa: E402

@@ -4,7 +4,8 @@
 * ``_brille*.so``: the pybind11 module with the host-side constructors
   (g++), linked against ``libbrille_amd.so`` with an ``$ORIGIN`` rpath.
 
-Usage: ``python -m brille_amd.build`` (or ``build()`` from Python).
+Usage: ``python brille_amd/build.py`` (the package refuses to import before
+its native library exists, so the script runs by path).
 """
 import os
 import subprocess
