@@ -146,7 +146,7 @@ def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
-            raise ImportError(f"{LIB_PATH} is missing: build it with `python -m brille_amd.build`")
+            raise ImportError(f"{LIB_PATH} is missing: build it with `python brille_amd/build.py`")
         L = C.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             f = getattr(L, name)
@@ -389,6 +389,30 @@ class Grid:
         if aux:
             return vals, vecs, bufs
         return vals, vecs
+
+    def run_device(self, Q, precision=FP64, kernel=KERNEL_AUTO, no_move=False, check=True, use_proposal=False,
+                   aux=("rot", "invrot", "node", "tet", "n_iw", "iw_index", "iw_weight", "status"),
+                   _entry="brille_ir_interpolate_at"):
+        """Device-memory call: Q and every output live in HBM for the call.  With
+        check=False no error is reduced and the per-point status is returned instead.
+        Returns (rc, values, vectors, aux dict) on the host."""
+        Q = np.ascontiguousarray(Q, dtype=np.float64).reshape(-1, 3)
+        n = Q.shape[0]
+        vs, cs = self.out_shapes(n)
+        dQ = DeviceArray.from_host(Q)
+        dv = DeviceArray(vs, np.float32 if precision == FP32 else np.float64)
+        dc = DeviceArray(cs, np.complex64 if precision == FP32 else np.complex128)
+        shapes = dict(ir_q=((n, 3), np.float64), tau=((n, 3), np.int32), rot=((n,), np.uint32),
+                      invrot=((n,), np.uint32), node=((n,), np.uint32), tet=((n,), np.uint32),
+                      n_iw=((n,), np.uint8), iw_index=((n, 8), np.uint32), iw_weight=((n, 8), np.float64),
+                      status=((n,), np.uint8))
+        dev = {k: DeviceArray(*shapes[k]) for k in aux}
+        a = AuxOut(**{k: v.ptr for k, v in dev.items()}) if dev else None
+        o = RunOpts(int(no_move), precision, int(use_proposal), MEM_DEVICE, kernel, int(check))
+        rc = getattr(lib(), _entry)(self.h, dQ.ptr, n, C.byref(o), dv.ptr, dc.ptr,
+                                    C.byref(a) if a is not None else None, None)
+        lib().brille_device_sync()
+        return rc, dv.to_host(), dc.to_host(), {k: v.to_host() for k, v in dev.items()}
 
     def ir_moveinto(self, Q):
         Q = np.ascontiguousarray(Q, dtype=np.float64).reshape(-1, 3)
