@@ -158,6 +158,18 @@ __global__ void k_batch_reset(BatchHeader* hdr, const BatchGrid* grids, uint32_t
   }
 }
 
+// a member grid as the batch kernel sees it
+static void describe_batch_grid(const brille_grid* G, BatchGrid& bg) {
+  std::memset(&bg, 0, sizeof(bg));
+  bg.g = G->g;
+  bg.d = G->d;
+  bg.err = G->err;
+  bg.kind = batch_kind(G->d);
+  const bool want_phase = G->d.vec_rot == BRILLE_ROT_GAMMA && G->d.ec[1] > 0;
+  bg.n_atoms = G->d.ec[1] / 3u;
+  bg.ph_in_lds = (bg.kind == BK_GAMMA3 || bg.kind == BK_GAMMA6 || (want_phase && bg.n_atoms <= 4)) ? 1 : 0;
+}
+
 struct Schedule {
   uint32_t tile = 256;       // points per tile = threads per block
   uint32_t blocks_per_cu = 4;
@@ -178,8 +190,12 @@ struct brille_batch {
   hipStream_t cap_stream = nullptr;
   // instantiated graphs by (precision, tile, blocks, grab)
   std::map<std::tuple<int, uint32_t, uint32_t, uint32_t>, hipGraphExec_t> graphs;
-  // host copy of d_grids, refreshed when a grid's tables change
+  // host copy of d_grids, refreshed when a grid's tables change, and the
+  // fill generation of each grid that copy was made from
   std::vector<brille_amd::BatchGrid> host_grids;
+  std::vector<uint64_t> fill_gen;
+  // the member grids' locks, taken for every run (address order)
+  std::vector<const brille_grid*> lock_set() const { return {grids.begin(), grids.end()}; }
 };
 
 namespace brille_amd {
@@ -277,6 +293,8 @@ static int check_args(brille_batch* b, const double* Q, const uint64_t* offsets,
     if (offsets[g + 1] < offsets[g]) return fail(BRILLE_ERR_INVALID_ARGUMENT, "batch offsets must not decrease");
     if (offsets[g + 1] > offsets[g] && (!Q || !vals[g] || !vecs[g]))
       return fail(BRILLE_ERR_INVALID_ARGUMENT, "null input or output pointer");
+    if (!b->grids[g]->filled)
+      return fail(BRILLE_ERR_NOT_FILLED, "grid " + std::to_string(g) + ": The interpolation data must be filled before interpolating.");
     int rc = check_rotation_modes(b->grids[g]);
     if (rc != BRILLE_OK) return rc;
   }
@@ -310,10 +328,18 @@ static int run_sequential(brille_batch* b, const double* Q, const uint64_t* offs
 
 // The wide factor tables of the wide-row grids (built on first use, the same
 // tables their own binned calls use); the grid copies on the device follow.
+// A member refilled since the batch last copied its descriptors (its payload
+// buffers were replaced) is described again; an unfilled one stops the run.
 static int ensure_batch_tables(brille_batch* b, int precision, hipStream_t s) {
   bool changed = false;
   for (size_t g = 0; g < b->grids.size(); ++g) {
     brille_grid* G = b->grids[g];
+    if (!G->filled) return fail(BRILLE_ERR_NOT_FILLED, "grid " + std::to_string(g) + ": The interpolation data must be filled before interpolating.");
+    if (b->fill_gen[g] != G->fill_gen) {
+      describe_batch_grid(G, b->host_grids[g]);
+      b->fill_gen[g] = G->fill_gen;
+      changed = true;
+    }
     if (!wide_rows(G->d)) continue;
     const void* before = precision == BRILLE_FP32 ? (const void*)G->d.wide_fac32 : (const void*)G->d.wide_fac64;
     const int rc = ensure_cube_rows(G, precision, s);
@@ -357,15 +383,7 @@ int brille_batch_create(brille_grid* const* grids, uint32_t n_grids, brille_batc
     if (!G) return fail(BRILLE_ERR_INVALID_ARGUMENT, "null grid");
     if (!G->filled) return fail(BRILLE_ERR_NOT_FILLED, "The interpolation data must be filled before interpolating.");
     if (G->g.n_nodes == 0) return fail(BRILLE_ERR_INVALID_ARGUMENT, "this handle holds no trellis");
-    BatchGrid& bg = host[i];
-    std::memset(&bg, 0, sizeof(bg));
-    bg.g = G->g;
-    bg.d = G->d;
-    bg.err = G->err;
-    bg.kind = batch_kind(G->d);
-    const bool want_phase = G->d.vec_rot == BRILLE_ROT_GAMMA && G->d.ec[1] > 0;
-    bg.n_atoms = G->d.ec[1] / 3u;
-    bg.ph_in_lds = (bg.kind == BK_GAMMA3 || bg.kind == BK_GAMMA6 || (want_phase && bg.n_atoms <= 4)) ? 1 : 0;
+    describe_batch_grid(G, host[i]);
   }
   auto* b = new brille_batch();
   b->grids.assign(grids, grids + n_grids);
@@ -389,6 +407,8 @@ int brille_batch_create(brille_grid* const* grids, uint32_t n_grids, brille_batc
   }
   b->d_grids = static_cast<BatchGrid*>(pg);
   b->host_grids = host;
+  b->fill_gen.resize(n_grids);
+  for (uint32_t i = 0; i < n_grids; ++i) b->fill_gen[i] = grids[i]->fill_gen;
   b->d_slots = static_cast<BatchSlot*>(ps);
   b->d_hdr = static_cast<BatchHeader*>(ph);
   b->cap_slots = n_grids;
@@ -398,6 +418,8 @@ int brille_batch_create(brille_grid* const* grids, uint32_t n_grids, brille_batc
 
 int brille_batch_run(brille_batch* b, const double* Q, const uint64_t* offsets, const brille_run_opts* opts,
                      void* const* values_out, void* const* vectors_out, void* stream) {
+  if (!b) return fail(BRILLE_ERR_INVALID_ARGUMENT, "null batch");
+  GridLocks member_locks(b->lock_set());  // the member grids stay as they are for the whole run
   brille_run_opts o{};
   o.check = 1;
   if (opts) o = *opts;
@@ -414,6 +436,8 @@ int brille_batch_run(brille_batch* b, const double* Q, const uint64_t* offsets, 
 
 int brille_batch_autotune(brille_batch* b, const double* Q, const uint64_t* offsets, const brille_run_opts* opts,
                           void* const* values_out, void* const* vectors_out, void* stream) {
+  if (!b) return fail(BRILLE_ERR_INVALID_ARGUMENT, "null batch");
+  GridLocks member_locks(b->lock_set());  // the member grids stay as they are for the whole run
   brille_run_opts o{};
   o.check = 1;
   if (opts) o = *opts;
@@ -476,6 +500,7 @@ int brille_batch_schedule(const brille_batch* b, uint32_t* tile_points, uint32_t
 
 int brille_batch_destroy(brille_batch* b) {
   if (!b) return BRILLE_OK;
+  { GridLocks wait_for_runs(b->lock_set()); }
   (void)hipDeviceSynchronize();
   for (auto& kv : b->graphs) (void)hipGraphExecDestroy(kv.second);
   if (b->d_grids) (void)hipFree(b->d_grids);
@@ -489,6 +514,8 @@ int brille_batch_destroy(brille_batch* b) {
 int brille_time_batch_run(brille_batch* b, const double* Q, const uint64_t* offsets, const brille_run_opts* opts,
                           void* const* values_out, void* const* vectors_out, uint32_t iters, void* stream,
                           brille_kernel_times* times) {
+  if (!b) return fail(BRILLE_ERR_INVALID_ARGUMENT, "null batch");
+  GridLocks member_locks(b->lock_set());  // the member grids stay as they are for the whole run
   if (!times) return fail(BRILLE_ERR_INVALID_ARGUMENT, "null times");
   hipStream_t s = static_cast<hipStream_t>(stream);
   brille_run_opts o{};

@@ -517,7 +517,7 @@ __global__ __launch_bounds__(256) void k_wide_factors(GridConst g, DataConst d, 
     v0[x] = h0 ? rc[l] : zero;
     v1[x] = h1 ? rc[l + 64] : zero;
   }
-  wide_antiphases<Real, 8>(v0[0], v1[0], v0, v1, e);
+  wide_antiphases<Real, 8>(v0[0], v1[0], v0, v1, e, sc);
   if (l == 0) {
 #pragma unroll
     for (int x = 0; x < 8; ++x)

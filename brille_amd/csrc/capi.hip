@@ -115,7 +115,7 @@ __global__ __launch_bounds__(256) void k_ir_moveinto(GridConst g, const double* 
   if (invrot_out) invrot_out[p] = ir;
   if (status_out) status_out[p] = (uint8_t)st;
   if (st != BRILLE_PT_OK) {
-    atomicMin(&err->first_bad, ((unsigned long long)(base + p) << 3) | (unsigned long long)st);
+    atomicMin(&err->first_bad, err_key(base + p, st));
     atomicAdd(&err->count, 1ull);
   }
 }
@@ -282,6 +282,7 @@ int brille_grid_create(const brille_grid_desc* desc, brille_grid** out) {
 
 int brille_grid_destroy(brille_grid* G) {
   if (!G) return BRILLE_OK;
+  { std::lock_guard<std::mutex> wait_for_calls(G->mu); }
   (void)hipDeviceSynchronize();
   free_all(G->grid_allocs);
   free_all(G->data_allocs);
@@ -295,6 +296,7 @@ int brille_grid_destroy(brille_grid* G) {
 
 int brille_grid_fill(brille_grid* G, const brille_data_desc* vals, const brille_data_desc* vecs,
                      const brille_gamma_desc* gamma, const brille_perm_desc* perm) {
+  BRILLE_LOCK_GRID(G);
   if (!G || !vals || !vecs) return fail(BRILLE_ERR_INVALID_ARGUMENT, "null argument");
   if (vals->n_vertices != G->n_vertices || vecs->n_vertices != G->n_vertices)
     return fail(BRILLE_ERR_INVALID_ARGUMENT, "Provided " + std::to_string(vals->n_vertices) + " arrays but " +
@@ -303,6 +305,21 @@ int brille_grid_fill(brille_grid* G, const brille_data_desc* vals, const brille_
     return fail(BRILLE_ERR_INVALID_ARGUMENT, "Inconsistent values and vectors provided to DualInterpolator");
   if (vals->is_complex || !vecs->is_complex)
     return fail(BRILLE_ERR_INVALID_ARGUMENT, "values must be real and vectors complex");
+  if (gamma) {
+    // the kernels store to output atom 3 F0 + i and read vectors[vector_index]:
+    // every entry is checked before anything is uploaded
+    const uint64_t m = (uint64_t)gamma->n_atoms * gamma->n_ops;
+    if (m && (!gamma->F0 || !gamma->vector_index))
+      return fail(BRILLE_ERR_INVALID_ARGUMENT, "GammaTable mappings must not be null");
+    if (gamma->n_vectors && !gamma->vectors)
+      return fail(BRILLE_ERR_INVALID_ARGUMENT, "GammaTable vectors must not be null");
+    for (uint64_t i = 0; i < m; ++i) {
+      if (gamma->F0[i] >= gamma->n_atoms)
+        return fail(BRILLE_ERR_INVALID_ARGUMENT, "GammaTable atom mapping out of range");
+      if (gamma->vector_index[i] >= gamma->n_vectors)
+        return fail(BRILLE_ERR_INVALID_ARGUMENT, "GammaTable vector index out of range");
+    }
+  }
   for (const brille_data_desc* x : {vals, vecs}) {
     if (x->elements[1] % 3) return fail(BRILLE_ERR_INVALID_ARGUMENT, "Vectors must have 3N elements per branch");
     if (x->elements[2] % 9) return fail(BRILLE_ERR_INVALID_ARGUMENT, "Matrices must have 9N elements per branch");
@@ -310,6 +327,8 @@ int brille_grid_fill(brille_grid* G, const brille_data_desc* vals, const brille_
       return fail(BRILLE_ERR_INVALID_ARGUMENT, "the elements per branch must not all be zero");
   }
   free_all(G->data_allocs);
+  ++G->fill_gen;  // the payload tables are replaced from here on (batches re-read them)
+  G->filled = false;
   DataConst& d = G->d;
   std::memset(&d, 0, sizeof(d));
   d.V = G->n_vertices;
@@ -370,6 +389,7 @@ int brille_grid_fill(brille_grid* G, const brille_data_desc* vals, const brille_
 }
 
 uint64_t brille_bytes_per_point(const brille_grid* G, int32_t precision) {
+  BRILLE_LOCK_GRID(G);
   if (!G || !G->filled) return 0;
   const uint64_t s = precision == BRILLE_FP32 ? 4 : 8;
   return (uint64_t)G->d.B * G->d.sv * s + (uint64_t)G->d.B * G->d.sc * 2 * s;
@@ -377,17 +397,20 @@ uint64_t brille_bytes_per_point(const brille_grid* G, int32_t precision) {
 
 int brille_ir_interpolate_at(brille_grid* G, const double* Q, uint64_t n, const brille_run_opts* opts_in,
                              void* values_out, void* vectors_out, const brille_aux_out* aux, void* stream) {
+  BRILLE_LOCK_GRID(G);
   return run_interpolate(G, Q, n, opts_in, values_out, vectors_out, aux, static_cast<hipStream_t>(stream), nullptr, 0);
 }
 
 int brille_interpolate_at(brille_grid* G, const double* Q, uint64_t n, const brille_run_opts* opts_in,
                           void* values_out, void* vectors_out, const brille_aux_out* aux, void* stream) {
+  BRILLE_LOCK_GRID(G);
   return run_interpolate(G, Q, n, opts_in, values_out, vectors_out, aux, static_cast<hipStream_t>(stream), nullptr, 0,
                          FOLD_FIRST_ZONE);
 }
 
 int brille_ir_moveinto(brille_grid* G, const double* Q, uint64_t n, int32_t memory, double* q_out, int32_t* tau_out,
                        uint32_t* rot_out, uint32_t* invrot_out, uint8_t* status_out, void* stream_v) {
+  BRILLE_LOCK_GRID(G);
   if (!G || (!Q && n)) return fail(BRILLE_ERR_INVALID_ARGUMENT, "null argument");
   if (n == 0) return BRILLE_OK;
   hipStream_t s = static_cast<hipStream_t>(stream_v);

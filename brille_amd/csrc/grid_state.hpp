@@ -2,6 +2,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -10,6 +12,15 @@
 #include "proposal.hpp"
 
 struct brille_grid {
+  // Every entry point that uses the handle holds `mu` for the whole call: the
+  // error record, staging and work buffers, side stream and proposal state are
+  // per handle, and the reference's ir_interpolate_at is const and re-entrant
+  // (src/bz_trellis.hpp:153-155), so calls from several host threads on one
+  // handle run one after the other and each gets its own results.
+  mutable std::mutex mu;
+  // bumped by every brille_grid_fill (its tables are replaced): a batch
+  // refreshes its copy of a member's descriptors when this changes
+  uint64_t fill_gen = 0;
   brille_amd::GridConst g{};
   brille_amd::DataConst d{};
   std::vector<void*> grid_allocs;
@@ -46,6 +57,26 @@ struct brille_grid {
 };
 
 namespace brille_amd {
+// the locks of several handles, taken in address order (no two callers can
+// wait on each other)
+class GridLocks {
+ public:
+  explicit GridLocks(std::vector<const brille_grid*> gs) {
+    std::sort(gs.begin(), gs.end());
+    gs.erase(std::unique(gs.begin(), gs.end()), gs.end());
+    for (const brille_grid* g : gs)
+      if (g) held_.emplace_back(g->mu);
+  }
+
+ private:
+  std::vector<std::unique_lock<std::mutex>> held_;
+};
+
+// holds the handle's lock for the rest of the enclosing scope (no-op for null)
+#define BRILLE_LOCK_GRID(G)                   \
+  std::unique_lock<std::mutex> grid_lock_;    \
+  if (G) grid_lock_ = std::unique_lock<std::mutex>((G)->mu)
+
 int fail(int code, const std::string& msg);
 const char* g_last_error_text();  // the message of the last failure on this thread
 int build_fast_tables(const brille_grid_desc* desc, brille_grid* G);

@@ -231,6 +231,7 @@ BZTrellisQdc::BZTrellisQdc(const BrillouinZone& b, double node_volume, bool alwa
 }
 
 void BZTrellisQdc::fill(Payload v, Payload c) {
+  std::lock_guard<std::mutex> lock(*host_mu);
   const size_t V = trellis.n_vertices();
   if (v.shape[0] != V || c.shape[0] != V)
     throw std::runtime_error("Provided " + std::to_string(v.shape[0]) + " arrays but " + std::to_string(V) + " were expected!");
@@ -269,6 +270,7 @@ brille_grid_desc BZTrellisQdc::describe() const {
 }
 
 brille_grid* BZTrellisQdc::handle() {
+  std::lock_guard<std::mutex> lock(*host_mu);
   if (!device) {
     brille_grid_desc d = describe();
     device = std::make_shared<DeviceGrid>();

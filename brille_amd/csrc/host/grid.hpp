@@ -6,6 +6,7 @@
 // and the GammaTable built per call (src/bz_trellis.hpp:182-186, src/phonon.hpp:106-167).
 #pragma once
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -66,6 +67,10 @@ class BZTrellisQdc {
   ZoneTables zone;
   std::shared_ptr<DeviceGrid> device;
   bool device_filled = false;
+  // guards the lazy device create + fill and the host payload it reads:
+  // callers release the GIL around interpolation, so several threads can
+  // reach handle() at once
+  std::shared_ptr<std::mutex> host_mu = std::make_shared<std::mutex>();
 
   BZTrellisQdc(const BrillouinZone& bz, double node_volume_fraction, bool always_triangulate, ApproxConfig cfg);
   void fill(Payload v, Payload c);

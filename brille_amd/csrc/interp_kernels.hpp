@@ -39,10 +39,21 @@ struct Cplx<float> {
   using V16 = float4;
 };
 
+// The first failure of a call, as the reference raises it: every point's
+// fold-in runs before any location (src/bz_trellis.hpp:164-168), moveinto's
+// check before the wedge search's (src/bz_move.cpp:158, 288-294), and a null
+// node is rethrown before the count of points not found
+// (src/trellis_poly.hpp:345-349).  The point statuses are numbered in that
+// order, so the key (status << kErrStatusShift) | index, minimised over the
+// failing points, gives the lowest status first and its lowest index.
+constexpr int kErrStatusShift = 59;
 struct ErrRec {
-  unsigned long long first_bad;  // (index << 3) | point status, min over failing points
+  unsigned long long first_bad;  // min over failing points of (status << kErrStatusShift) | index
   unsigned long long count;
 };
+__host__ __device__ __forceinline__ unsigned long long err_key(uint64_t index, int status) {
+  return ((unsigned long long)status << kErrStatusShift) | (unsigned long long)index;
+}
 
 // Phase 1 result of one point, held in registers.
 struct Located {
@@ -867,27 +878,69 @@ __device__ __forceinline__ const double2* wide_factors<double>(const DataConst& 
 // each row (p0/p1 of the pivot, v0/v1 of the slots); the Hermitian products
 // are wave sums, all slots side by side.  One definition for the sums and
 // for the factor table, so both give the same bits.
+//
+// FP32 reduces across the wave (a tree).  FP64 keeps the reference's order
+// (src/utilities.tpp:567-577: one term per element, added to the running sum
+// element by element): where two modes are degenerate their overlap is
+// rounding noise around zero, and its antiphase is then decided by the exact
+// order of the additions.  Each lane forms its elements' terms; the sums take
+// them lane by lane with uniform readlane broadcasts.
+__device__ __forceinline__ double readlane_f64(double v, uint32_t k) {
+  const unsigned long long u = __double_as_longlong(v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, (int)k);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), (int)k);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 template <typename Real, int NS>
 __device__ __forceinline__ void wide_antiphases(typename Cplx<Real>::T p0, typename Cplx<Real>::T p1,
                                                 const typename Cplx<Real>::T* v0, const typename Cplx<Real>::T* v1,
-                                                typename Cplx<Real>::T* e) {
-  Real re[NS], im[NS];
-#pragma unroll
-  for (int x = 0; x < NS; ++x) {
-    re[x] = p0.x * v0[x].x + p0.y * v0[x].y;
-    re[x] += p1.x * v1[x].x + p1.y * v1[x].y;
-    im[x] = p0.x * v0[x].y - p0.y * v0[x].x;
-    im[x] += p1.x * v1[x].y - p1.y * v1[x].x;
-  }
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1)
+                                                typename Cplx<Real>::T* e, uint32_t sc) {
+  if constexpr (sizeof(Real) == 8) {
+    Real re0[NS], im0[NS], re1[NS], im1[NS], re[NS], im[NS];
 #pragma unroll
     for (int x = 0; x < NS; ++x) {
-      re[x] += __shfl_xor(re[x], o);
-      im[x] += __shfl_xor(im[x], o);
+      re0[x] = p0.x * v0[x].x + p0.y * v0[x].y;
+      im0[x] = p0.x * v0[x].y - p0.y * v0[x].x;
+      re1[x] = p1.x * v1[x].x + p1.y * v1[x].y;
+      im1[x] = p1.x * v1[x].y - p1.y * v1[x].x;
+      re[x] = 0;
+      im[x] = 0;
+    }
+    const uint32_t n0 = sc < 64u ? sc : 64u;
+    for (uint32_t k = 0; k < n0; ++k)
+#pragma unroll
+      for (int x = 0; x < NS; ++x) {
+        re[x] += readlane_f64(re0[x], k);
+        im[x] += readlane_f64(im0[x], k);
+      }
+    for (uint32_t k = 64; k < sc; ++k)
+#pragma unroll
+      for (int x = 0; x < NS; ++x) {
+        re[x] += readlane_f64(re1[x], k - 64u);
+        im[x] += readlane_f64(im1[x], k - 64u);
+      }
+#pragma unroll
+    for (int x = 0; x < NS; ++x) e[x] = antiphase_closed<Real>(re[x], im[x]);
+  } else {
+    Real re[NS], im[NS];
+#pragma unroll
+    for (int x = 0; x < NS; ++x) {
+      re[x] = p0.x * v0[x].x + p0.y * v0[x].y;
+      re[x] += p1.x * v1[x].x + p1.y * v1[x].y;
+      im[x] = p0.x * v0[x].y - p0.y * v0[x].x;
+      im[x] += p1.x * v1[x].y - p1.y * v1[x].x;
     }
 #pragma unroll
-  for (int x = 0; x < NS; ++x) e[x] = antiphase_closed<Real>(re[x], im[x]);
+    for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+      for (int x = 0; x < NS; ++x) {
+        re[x] += __shfl_xor(re[x], o);
+        im[x] += __shfl_xor(im[x], o);
+      }
+#pragma unroll
+    for (int x = 0; x < NS; ++x) e[x] = antiphase_closed<Real>(re[x], im[x]);
+  }
 }
 
 // What a lane's Γ rotation needs of a point, kept while a wave stays on it
@@ -932,7 +985,7 @@ __device__ inline void unit_wide(const GridConst& g, const DataConst& d, const P
   } else {
     const C* prow = reinterpret_cast<const C*>(row_at<Real>(d, r.pivot, b));
     const C p0 = h0 ? prow[l] : zero, p1 = h1 ? prow[l + 64] : zero;
-    wide_antiphases<Real, 8>(p0, p1, v0, v1, e);
+    wide_antiphases<Real, 8>(p0, p1, v0, v1, e, sc);
   }
   C a0 = zero, a1 = zero;
   Real va = 0;
@@ -1327,7 +1380,7 @@ __device__ __forceinline__ void tile_sums(const GridConst& g, const DataConst& d
 }
 
 __device__ __forceinline__ void report_point(ErrRec* err, uint64_t index, int status) {
-  atomicMin(&err->first_bad, (index << 3) | (unsigned long long)status);
+  atomicMin(&err->first_bad, err_key(index, status));
   atomicAdd(&err->count, 1ull);
 }
 

@@ -395,6 +395,7 @@ static void grads_launch(brille_grid* G, const StepBuffers& sb, const float* fea
 extern "C" {
 
 int brille_proposal_train(brille_grid* G, const brille_train_opts* opts_in, brille_train_report* report) {
+  BRILLE_LOCK_GRID(G);
   if (!G) return fail(BRILLE_ERR_INVALID_ARGUMENT, "null grid");
   brille_train_opts o{};
   if (opts_in) o = *opts_in;
@@ -514,6 +515,7 @@ int brille_proposal_train(brille_grid* G, const brille_train_opts* opts_in, bril
 }
 
 int brille_proposal_stats(const brille_grid* G, uint64_t* tried, uint64_t* hits) {
+  BRILLE_LOCK_GRID(G);
   if (!G) return fail(BRILLE_ERR_INVALID_ARGUMENT, "null grid");
   if (tried) *tried = G->last_tried;
   if (hits) *hits = G->last_hits;
@@ -522,6 +524,7 @@ int brille_proposal_stats(const brille_grid* G, uint64_t* tried, uint64_t* hits)
 
 int brille_proposal_loss_grad(brille_grid* G, const float* x, const int32_t* labels, uint32_t n, float* loss,
                               float* grads, uint32_t* n_params) {
+  BRILLE_LOCK_GRID(G);
   if (!G || !x || !labels || n == 0) return fail(BRILLE_ERR_INVALID_ARGUMENT, "null argument");
   if (!G->prop_params) {
     int rc = prop_setup(G, 0);
@@ -575,6 +578,7 @@ int brille_proposal_loss_grad(brille_grid* G, const float* x, const int32_t* lab
 }
 
 int brille_proposal_get_params(const brille_grid* G, float* params, uint32_t* n_params) {
+  BRILLE_LOCK_GRID(G);
   if (!G) return fail(BRILLE_ERR_INVALID_ARGUMENT, "null grid");
   if (!G->prop_params) return fail(BRILLE_ERR_INVALID_ARGUMENT, "the proposal has no parameters yet");
   if (n_params) *n_params = (uint32_t)G->prop_n_params;
@@ -584,6 +588,7 @@ int brille_proposal_get_params(const brille_grid* G, float* params, uint32_t* n_
 }
 
 int brille_proposal_set_params(brille_grid* G, const float* params, uint32_t n_params) {
+  BRILLE_LOCK_GRID(G);
   if (!G || !params) return fail(BRILLE_ERR_INVALID_ARGUMENT, "null argument");
   if (!G->prop_params) {
     int rc = prop_setup(G, 0);
@@ -597,6 +602,7 @@ int brille_proposal_set_params(brille_grid* G, const float* params, uint32_t n_p
 }
 
 int brille_proposal_predict(brille_grid* G, const float* x, const int32_t* in_tet, uint32_t n, int32_t* out) {
+  BRILLE_LOCK_GRID(G);
   if (!G || !x || !in_tet || !out) return fail(BRILLE_ERR_INVALID_ARGUMENT, "null argument");
   if (!G->prop_trained) return fail(BRILLE_ERR_INVALID_ARGUMENT, "the proposal has not been trained or set");
   if (n == 0) return BRILLE_OK;

@@ -141,8 +141,8 @@ static int dispatch(brille_grid* G, const LaunchArgs& a, const brille_run_opts& 
 }
 
 int report_point_error(const ErrRec& e, const double* Q, uint64_t n, bool host_mem) {
-  const uint64_t idx = e.first_bad >> 3;
-  const int st = (int)(e.first_bad & 7ull);
+  const uint64_t idx = e.first_bad & ((1ull << kErrStatusShift) - 1ull);
+  const int st = (int)(e.first_bad >> kErrStatusShift);
   double row[3];
   const double* rp = nullptr;
   if (idx < n) {
@@ -381,6 +381,7 @@ extern "C" {
 int brille_time_ir_interpolate_at(brille_grid* G, const double* Q, uint64_t n, const brille_run_opts* opts,
                                   void* values_out, void* vectors_out, uint32_t iters, void* stream,
                                   brille_kernel_times* times) {
+  BRILLE_LOCK_GRID(G);
   if (!times) return fail(BRILLE_ERR_INVALID_ARGUMENT, "null times");
   if (opts && opts->memory != BRILLE_MEM_DEVICE) return fail(BRILLE_ERR_INVALID_ARGUMENT, "timing needs device memory");
   return run_interpolate(G, Q, n, opts, values_out, vectors_out, nullptr, static_cast<hipStream_t>(stream), times, iters);

@@ -741,16 +741,20 @@ int brille_oracle_ir_interpolate_at(const brille_grid_desc* g, const brille_data
     }
   }
 
-  for (uint64_t i = 0; i < n; ++i) {
-    if (status[i] != BRILLE_PT_OK) {
-      bad = i;
-      switch (status[i]) {
-        case BRILLE_PT_OUTSIDE_BZ: result = BRILLE_ERR_OUTSIDE_BZ; break;
-        case BRILLE_PT_OUTSIDE_IR: result = BRILLE_ERR_OUTSIDE_IR; break;
-        case BRILLE_PT_NULL_NODE: result = BRILLE_ERR_NULL_NODE; break;
-        default: result = BRILLE_ERR_NOT_FOUND;
-      }
-      break;
+  /* The reference raises the first failure of the earliest stage: moveinto's
+   * check over all points (src/bz_move.cpp:158), then the wedge search's first
+   * point (:288-294), then a null node (rethrown, src/trellis_poly.hpp:341),
+   * then the count of points not found (:345-349).  The point statuses are
+   * numbered in that order: the lowest status wins, then the lowest index. */
+  uint8_t worst = 255;
+  for (uint64_t i = 0; i < n; ++i)
+    if (status[i] != BRILLE_PT_OK && status[i] < worst) { worst = status[i]; bad = i; }
+  if (worst != 255) {
+    switch (worst) {
+      case BRILLE_PT_OUTSIDE_BZ: result = BRILLE_ERR_OUTSIDE_BZ; break;
+      case BRILLE_PT_OUTSIDE_IR: result = BRILLE_ERR_OUTSIDE_IR; break;
+      case BRILLE_PT_NULL_NODE: result = BRILLE_ERR_NULL_NODE; break;
+      default: result = BRILLE_ERR_NOT_FOUND;
     }
   }
   if (result == BRILLE_OK && rot_status != BRILLE_OK) result = rot_status;

@@ -142,3 +142,37 @@ def test_batch_reports_failing_grid(small_batch):
     msg = buf.value.decode()
     assert msg.startswith(f"grid {g}:"), msg
     assert np.all(vals[g].to_host()[5] == 0)
+
+
+def test_batch_follows_a_refilled_member(small_batch):
+    """brille_grid_fill replaces a member's payload buffers: the next batch run describes that
+    grid again (its fill generation changed) and gives what a per-grid call on the new payload
+    gives; an emptied member stops the run instead of reading freed memory."""
+    grids, tables, devs, Q, offsets = small_batch
+    say("batch after a member is refilled")
+    t0 = dict(tables[0])
+    d0 = capi.Grid(t0)
+    batch = capi.Batch([d0, devs[1]])
+    off = np.array([0, 3000, 5000], np.uint64)
+    Qs = Q[:5000]
+    dQ = capi.DeviceArray.from_host(Qs)
+    vals, vecs = batch.outputs(off, capi.FP32)
+    assert batch.run(dQ, off, vals, vecs, precision=capi.FP32) == 0
+    before = vals[0].to_host()
+    # refill grid 0 with scaled values and conjugated vectors
+    t1 = dict(t0)
+    t1["values"] = 2.5 * np.asarray(t0["values"])
+    t1["vectors"] = np.conj(np.asarray(t0["vectors"]))
+    d0.fill(t1)
+    assert batch.run(dQ, off, vals, vecs, precision=capi.FP32) == 0
+    v, c = d0.ir_interpolate_at(Qs[:3000], precision=capi.FP32, kernel=capi.KERNEL_TILE)
+    np.testing.assert_array_equal(vals[0].to_host(), v)
+    np.testing.assert_array_equal(vecs[0].to_host(), c)
+    assert not np.array_equal(before, v)
+    # a failed fill leaves the member unfilled: the batch reports it
+    bad = dict(t0)
+    bad["values"] = np.asarray(t0["values"])[:10]
+    with pytest.raises(capi.BrilleError):
+        d0.fill(bad)
+    rc = batch.run(dQ, off, vals, vecs, precision=capi.FP32)
+    assert capi.STATUS_NAMES[rc] == "NOT_FILLED"

@@ -221,3 +221,22 @@ def test_first_zone_interpolate_at(kernel):
         dev.interpolate_at(Qb, precision=capi.FP32, kernel=kernel)
     assert e.value.status == "NULL_NODE"
     assert f"{Qb[bad][0]:.17g}"[:12] in str(e.value) or str(bad) in str(e.value)
+
+
+@pytest.mark.parametrize("kernel", [capi.KERNEL_PLAIN, capi.KERNEL_TILE, capi.KERNEL_BINNED])
+def test_earlier_stage_errors_win(kernel):
+    """A fold-in failure (NaN: outside the zone) at a later index is the call's error even when a
+    null node comes first, as the reference folds every point before locating any
+    (src/bz_trellis.hpp:164-168)."""
+    say(f"error precedence kernel={kernel}")
+    g = S.grid(1)
+    t = g.tables()
+    dev = capi.Grid(t)
+    Qb = S.q_points(1, 3000, stream=42)
+    rc, *_, bad = O.ir_interpolate_at(t, Qb, first_zone=True)
+    Q = np.vstack([Qb[bad:bad + 1], Qb[:50], [[np.nan, 0.0, 0.0]], Qb[:5]])
+    orc, *_, obad = O.ir_interpolate_at(t, Q, first_zone=True)
+    assert orc == 3 and obad == 51
+    with pytest.raises(capi.BrilleError) as e:
+        dev.interpolate_at(Q, precision=capi.FP32, kernel=kernel)
+    assert e.value.status == "OUTSIDE_BZ" and "nan" in str(e.value)

@@ -268,3 +268,19 @@ def test_first_zone_mode_fails_outside_the_wedge():
     rc, v, c, aux, bad = O.ir_interpolate_at(t, Q, first_zone=True)
     assert rc == 5  # BRILLE_ERR_NULL_NODE: first-zone images outside the trellis
     assert aux["status"][bad] == 3  # BRILLE_PT_NULL_NODE
+
+
+def test_earlier_stage_errors_win_over_earlier_indices():
+    """The reference folds every point before it locates any (src/bz_trellis.hpp:164-168) and
+    checks moveinto before the wedge (src/bz_move.cpp:158, 288-294): a stage-(a) failure at a
+    later index is the error even when a stage-(b) failure comes first in the input."""
+    from brille_amd import synthetic as S
+    g = S.grid(1)
+    t = g.tables()
+    Qb = S.q_points(1, 3000, stream=42)
+    rc, *_, bad = O.ir_interpolate_at(t, Qb, first_zone=True)
+    assert rc == 5  # first-zone images outside the wedge: null nodes
+    Q = np.vstack([Qb[bad:bad + 1], Qb[:50], [[np.nan, 0.0, 0.0]], Qb[:5]])
+    rc, v, c, aux, bad2 = O.ir_interpolate_at(t, Q, first_zone=True)
+    assert aux["status"][0] == 3 and aux["status"][51] == 1
+    assert rc == 3 and bad2 == 51  # BRILLE_ERR_OUTSIDE_BZ at the NaN, not the null node at 0
