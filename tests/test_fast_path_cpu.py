@@ -100,3 +100,40 @@ def test_fast_equals_exact_nacl(harness, nacl):
     Q = np.vstack([q_nu, rng.uniform(-2, 2, size=(50000, 3)), special_points(g, 0, rng)])
     q, tau, r, ir, st, nodes, counts = run(harness, t, Q)
     assert counts[0] == 0
+
+
+def _near_boundaries(g, rng, scales):
+    """Zone vertices, face points and wedge-plane points moved just outside and inside by each
+    relative amount in `scales`, and their symmetry images shifted by lattice vectors."""
+    bz = g.BrillouinZone
+    t = g.tables()
+    verts = np.asarray(bz.vertices)
+    faces = np.asarray(bz.points)
+    ir = np.asarray(bz.ir_vertices)
+    mids = (ir[:, None] + ir[None, :]).reshape(-1, 3) / 2
+    base = np.vstack([verts, faces, (verts[:, None] + verts[None, :]).reshape(-1, 3) / 2, mids])
+    out = []
+    for s in scales:
+        out += [base * (1 + s), base * (1 - s)]
+        d = rng.normal(size=base.shape)
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+        out.append(base + s * d)
+    pts = np.vstack(out)
+    Rs = t["rotations"].reshape(-1, 3, 3)
+    imgs = [pts] + [pts @ Rs[j] + rng.integers(-2, 3, size=(1, 3)) for j in range(0, len(Rs), max(1, len(Rs) // 8))]
+    return np.vstack(imgs)
+
+
+@pytest.mark.parametrize("tol", [1e-10, 1e-8, 1e-6, 1e-4])
+@pytest.mark.parametrize("cfg", [0, 1])
+def test_fast_equals_exact_with_a_loose_zone_tolerance(harness, cfg, tol):
+    """ApproxConfig's reciprocal-space tolerance widens the exact tests' zero band; the fast
+    margins follow it, so points just outside faces and wedge planes are still decided alike."""
+    g = S.grid(cfg)
+    t = dict(g.tables())
+    t["float_tolerance"] = tol
+    rng = np.random.default_rng(int(-np.log10(tol)) + 10 * cfg)
+    scales = np.concatenate([tol * np.array([0.3, 0.6, 0.9, 1.0, 1.1, 1.5, 2, 3, 10]), np.logspace(-12, -3, 10)])
+    Q = np.vstack([S.q_points(cfg, 20000, stream=23), _near_boundaries(g, rng, scales)])
+    q, tau, r, ir, st, nodes, counts = run(harness, t, Q)
+    assert counts[0] == 0, f"{counts[0]} of {len(Q)} points differ at tolerance {tol}"
