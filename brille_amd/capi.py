@@ -19,6 +19,7 @@ STATUS_NAMES = {
 }
 FP64, FP32 = 0, 1
 MEM_DEVICE, MEM_HOST = 0, 1
+KEEP_BOTH, KEEP_FP64, KEEP_FP32 = 0, 1, 2
 KERNEL_AUTO, KERNEL_PLAIN, KERNEL_TUNED, KERNEL_TILE, KERNEL_BINNED = 0, 1, 2, 3, 4
 
 _dp = C.POINTER(C.c_double)
@@ -67,6 +68,10 @@ class PermDesc(C.Structure):
                 ("perms", _up)]
 
 
+class FillOpts(C.Structure):
+    _fields_ = [("memory", C.c_int32), ("data_type", C.c_int32), ("keep", C.c_int32)]
+
+
 class RunOpts(C.Structure):
     _fields_ = [("no_move", C.c_int32), ("precision", C.c_int32), ("use_proposal", C.c_int32),
                 ("memory", C.c_int32), ("kernel", C.c_int32), ("check", C.c_int32)]
@@ -102,6 +107,8 @@ SIGNATURES = {
     "brille_grid_fill": (C.c_int, [C.c_void_p, C.POINTER(DataDesc), C.POINTER(DataDesc), C.POINTER(GammaDesc),
                                    C.POINTER(PermDesc)]),
     "brille_grid_destroy": (C.c_int, [C.c_void_p]),
+    "brille_grid_fill_ex": (C.c_int, [C.c_void_p, C.POINTER(DataDesc), C.POINTER(DataDesc), C.POINTER(GammaDesc),
+                                      C.POINTER(PermDesc), C.POINTER(FillOpts)]),
     "brille_bytes_per_point": (C.c_uint64, [C.c_void_p, C.c_int32]),
     "brille_ir_interpolate_at": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(RunOpts), C.c_void_p,
                                            C.c_void_p, C.POINTER(AuxOut), C.c_void_p]),
@@ -348,6 +355,31 @@ class Grid:
                                      C.byref(d.gdesc) if d.gdesc is not None else None,
                                      C.byref(d.pdesc) if d.pdesc is not None else None))
 
+    def fill_ex(self, tables, values_ptr, vectors_ptr, memory, data_type, keep=KEEP_BOTH, perm=None):
+        """brille_grid_fill_ex: the payload's element layout and GammaTable come from `tables`
+        (its own values/vectors arrays are not read); values_ptr/vectors_ptr point to the
+        payload, float/complex64 when data_type is FP32, in host or device memory."""
+        V = int(np.asarray(tables["vertices"]).shape[0])
+        B = int(tables["branches"])
+        sv = int(np.sum(tables["values_elements"]))
+        sc = int(np.sum(tables["vectors_elements"]))
+        shape_v = (V, B, sv)
+        shape_c = (V, B, sc)
+        stub = dict(tables)
+        stub["values"] = np.zeros((1,) + shape_v[1:])
+        stub["vectors"] = np.zeros((1,) + shape_c[1:], complex)
+        d = DataHolder(stub, perm=perm)
+        for desc, ptr in ((d.vdesc, values_ptr), (d.cdesc, vectors_ptr)):
+            desc.n_vertices = V
+            desc.data = C.cast(C.c_void_p(int(ptr)), _dp)
+        self.data = d
+        self.data.vals = np.empty(shape_v, np.float32)[:0]  # shapes for out_shapes
+        self.data.vals_shape, self.data.vecs_shape = shape_v, shape_c
+        o = FillOpts(memory, data_type, keep)
+        check(lib().brille_grid_fill_ex(self.h, C.byref(d.vdesc), C.byref(d.cdesc),
+                                        C.byref(d.gdesc) if d.gdesc is not None else None,
+                                        C.byref(d.pdesc) if d.pdesc is not None else None, C.byref(o)))
+
     @property
     def branches(self):
         return self.data.vdesc.branches
@@ -355,8 +387,8 @@ class Grid:
     def out_shapes(self, n):
         if self.data is None:  # unfilled: the library reports it before touching the outputs
             return (n, 0), (n, 0)
-        vs = (n,) + self.data.vals.shape[1:]
-        cs = (n,) + self.data.vecs.shape[1:]
+        vs = (n,) + tuple(getattr(self.data, "vals_shape", self.data.vals.shape)[1:])
+        cs = (n,) + tuple(getattr(self.data, "vecs_shape", self.data.vecs.shape)[1:])
         return vs, cs
 
     def interpolate_at(self, Q, precision=FP64, kernel=KERNEL_AUTO, no_move=False, aux=False):

@@ -296,7 +296,8 @@ static int check_args(brille_batch* b, const double* Q, const uint64_t* offsets,
     if (!b->grids[g]->filled)
       return fail(BRILLE_ERR_NOT_FILLED, "grid " + std::to_string(g) + ": The interpolation data must be filled before interpolating.");
     int rc = check_rotation_modes(b->grids[g]);
-    if (rc != BRILLE_OK) return rc;
+    if (rc == BRILLE_OK) rc = check_payload_precision(b->grids[g], o.precision);
+    if (rc != BRILLE_OK) return fail(rc, "grid " + std::to_string(g) + ": " + std::string(g_last_error_text()));
   }
   return BRILLE_OK;
 }
@@ -498,9 +499,9 @@ int brille_batch_schedule(const brille_batch* b, uint32_t* tile_points, uint32_t
   return BRILLE_OK;
 }
 
+// touches only what the batch owns: its member grids may already be gone
 int brille_batch_destroy(brille_batch* b) {
   if (!b) return BRILLE_OK;
-  { GridLocks wait_for_runs(b->lock_set()); }
   (void)hipDeviceSynchronize();
   for (auto& kv : b->graphs) (void)hipGraphExecDestroy(kv.second);
   if (b->d_grids) (void)hipFree(b->d_grids);

@@ -63,18 +63,49 @@ static uint32_t row_stride(uint32_t used, size_t real_bytes) {
   return (uint32_t)(bytes / real_bytes);
 }
 
-// rows of (vertex, branch): vectors (re, im interleaved) then values, zero pad
-static void build_rows(const DataConst& d, const double* vals, const double* vecs, uint32_t stride,
-                       std::vector<double>& out) {
-  const size_t R = (size_t)d.V * d.B;
-  out.assign(R * stride, 0.0);
-  for (size_t r = 0; r < R; ++r) {
-    double* o = out.data() + r * stride;
-    const double* c = vecs + r * 2 * d.sc;
-    for (uint32_t k = 0; k < 2 * d.sc; ++k) o[k] = c[k];
-    const double* v = vals + r * d.sv;
-    for (uint32_t k = 0; k < d.sv; ++k) o[2 * d.sc + k] = v[k];
+// Packing of payload rows: row r = (vertex, branch) holds the vectors'
+// complex elements (re, im interleaved) then the values, zero padded to the
+// stride.  One thread per output real; In -> Out converts by rounding to
+// nearest (a static_cast).
+constexpr uint64_t kFillPiece = 256ull << 20;  // bytes of caller data staged per packing launch
+
+template <typename In, typename Out>
+__global__ __launch_bounds__(256) void k_pack_rows(const In* __restrict__ vals, const In* __restrict__ vecs,
+                                                   uint64_t m, uint32_t sv, uint32_t sc, uint32_t stride,
+                                                   Out* __restrict__ out) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= m * stride) return;
+  const uint64_t r = t / stride;
+  const uint32_t k = (uint32_t)(t - r * stride);
+  In x = (In)0;
+  if (k < 2 * sc) x = vecs[r * 2 * sc + k];
+  else if (k < 2 * sc + sv) x = vals[r * sv + (k - 2 * sc)];
+  out[t] = static_cast<Out>(x);
+}
+
+template <typename In>
+static void pack_both(const In* v, const In* c, uint64_t m, const DataConst& d, double* o64, float* o32) {
+  if (o64) {
+    const uint64_t n = m * d.stride64;
+    hipLaunchKernelGGL((k_pack_rows<In, double>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, v, c, m, d.sv,
+                       d.sc, d.stride64, o64);
   }
+  if (o32) {
+    const uint64_t n = m * d.stride32;
+    hipLaunchKernelGGL((k_pack_rows<In, float>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, v, c, m, d.sv,
+                       d.sc, d.stride32, o32);
+  }
+}
+
+// rows [r0, r0 + m) from values v and vectors c (device memory, data_type reals)
+static int pack_rows(const DataConst& d, const void* v, const void* c, int data_type, uint64_t r0, uint64_t m,
+                     double* rows64, float* rows32) {
+  double* o64 = rows64 ? rows64 + r0 * d.stride64 : nullptr;
+  float* o32 = rows32 ? rows32 + r0 * d.stride32 : nullptr;
+  if (data_type == BRILLE_FP32) pack_both(static_cast<const float*>(v), static_cast<const float*>(c), m, d, o64, o32);
+  else pack_both(static_cast<const double*>(v), static_cast<const double*>(c), m, d, o64, o32);
+  if (hipGetLastError() != hipSuccess) return fail(BRILLE_ERR_DEVICE, "payload packing launch failed");
+  return BRILLE_OK;
 }
 
 static void make_tols(double Ttol, int tol, double* rel, double* abs_) {
@@ -294,9 +325,63 @@ int brille_grid_destroy(brille_grid* G) {
   return BRILLE_OK;
 }
 
-int brille_grid_fill(brille_grid* G, const brille_data_desc* vals, const brille_data_desc* vecs,
-                     const brille_gamma_desc* gamma, const brille_perm_desc* perm) {
-  BRILLE_LOCK_GRID(G);
+// The payload is packed into rows on the device: the caller's values (V, B,
+// sv) and complex vectors (V, B, sc), host or device, FP64 or FP32, go
+// through packing kernels in pieces of at most kFillPiece bytes of input, so
+// no host copy of the rows is ever made (the largest payloads are tens of GB).
+static int fill_payload(brille_grid* G, const brille_data_desc* vals, const brille_data_desc* vecs,
+                        const brille_fill_opts& o) {
+  DataConst& d = G->d;
+  auto& A = G->data_allocs;
+  const uint64_t R = (uint64_t)d.V * d.B;
+  const size_t in_real = o.data_type == BRILLE_FP32 ? 4 : 8;
+  const bool keep64 = o.keep == 0 || (o.keep & BRILLE_KEEP_FP64);
+  const bool keep32 = o.keep == 0 || (o.keep & BRILLE_KEEP_FP32);
+  void* out64 = nullptr;
+  void* out32 = nullptr;
+  if (keep64) {
+    HIP_TRY(hipMalloc(&out64, R * d.stride64 * sizeof(double) + 16));
+    A.push_back(out64);
+    d.rows64 = static_cast<const double*>(out64);
+  }
+  if (keep32) {
+    HIP_TRY(hipMalloc(&out32, R * d.stride32 * sizeof(float) + 16));
+    A.push_back(out32);
+    d.rows32 = static_cast<const float*>(out32);
+  }
+  if (R == 0) return BRILLE_OK;
+  const size_t row_in = (size_t)(2 * d.sc + d.sv) * in_real;
+  const uint64_t piece = std::max<uint64_t>(1, kFillPiece / row_in);
+  const char* hv = reinterpret_cast<const char*>(vals->data);
+  const char* hc = reinterpret_cast<const char*>(vecs->data);
+  void* stage = nullptr;
+  if (o.memory == BRILLE_MEM_HOST) HIP_TRY(hipMalloc(&stage, std::min<uint64_t>(piece, R) * row_in + 256));
+  int rc = BRILLE_OK;
+  for (uint64_t r0 = 0; r0 < R && rc == BRILLE_OK; r0 += piece) {
+    const uint64_t m = std::min(piece, R - r0);
+    const size_t vb = (size_t)m * d.sv * in_real, cb = (size_t)m * 2 * d.sc * in_real;
+    const void* pv = hv + r0 * d.sv * in_real;
+    const void* pc = hc + r0 * 2 * d.sc * in_real;
+    if (o.memory == BRILLE_MEM_HOST) {
+      char* st = static_cast<char*>(stage);
+      if (hipMemcpy(st, pc, cb, hipMemcpyHostToDevice) != hipSuccess ||
+          hipMemcpy(st + ((cb + 255) & ~size_t(255)), pv, vb, hipMemcpyHostToDevice) != hipSuccess) {
+        rc = fail(BRILLE_ERR_DEVICE, "hipMemcpy failed while filling the payload");
+        break;
+      }
+      pc = st;
+      pv = st + ((cb + 255) & ~size_t(255));
+    }
+    rc = pack_rows(d, pv, pc, o.data_type, r0, m, static_cast<double*>(out64), static_cast<float*>(out32));
+  }
+  if (stage) (void)hipFree(stage);
+  if (rc == BRILLE_OK && hipDeviceSynchronize() != hipSuccess) rc = fail(BRILLE_ERR_DEVICE, "payload packing failed");
+  return rc;
+}
+
+static int grid_fill(brille_grid* G, const brille_data_desc* vals, const brille_data_desc* vecs,
+                     const brille_gamma_desc* gamma, const brille_perm_desc* perm, const brille_fill_opts& o) {
+  // every check first: a fill that fails them leaves the previous payload in place
   if (!G || !vals || !vecs) return fail(BRILLE_ERR_INVALID_ARGUMENT, "null argument");
   if (vals->n_vertices != G->n_vertices || vecs->n_vertices != G->n_vertices)
     return fail(BRILLE_ERR_INVALID_ARGUMENT, "Provided " + std::to_string(vals->n_vertices) + " arrays but " +
@@ -305,9 +390,23 @@ int brille_grid_fill(brille_grid* G, const brille_data_desc* vals, const brille_
     return fail(BRILLE_ERR_INVALID_ARGUMENT, "Inconsistent values and vectors provided to DualInterpolator");
   if (vals->is_complex || !vecs->is_complex)
     return fail(BRILLE_ERR_INVALID_ARGUMENT, "values must be real and vectors complex");
+  if (o.memory != BRILLE_MEM_HOST && o.memory != BRILLE_MEM_DEVICE)
+    return fail(BRILLE_ERR_INVALID_ARGUMENT, "unknown payload memory");
+  if (o.data_type != BRILLE_FP64 && o.data_type != BRILLE_FP32)
+    return fail(BRILLE_ERR_INVALID_ARGUMENT, "unknown payload data type");
+  if (o.keep & ~(BRILLE_KEEP_FP64 | BRILLE_KEEP_FP32))
+    return fail(BRILLE_ERR_INVALID_ARGUMENT, "unknown payload precision to keep");
+  for (const brille_data_desc* x : {vals, vecs}) {
+    if (x->elements[1] % 3) return fail(BRILLE_ERR_INVALID_ARGUMENT, "Vectors must have 3N elements per branch");
+    if (x->elements[2] % 9) return fail(BRILLE_ERR_INVALID_ARGUMENT, "Matrices must have 9N elements per branch");
+    if (x->elements[0] + x->elements[1] + x->elements[2] == 0)
+      return fail(BRILLE_ERR_INVALID_ARGUMENT, "the elements per branch must not all be zero");
+    if (G->n_vertices && !x->data) return fail(BRILLE_ERR_INVALID_ARGUMENT, "null payload data");
+  }
   if (gamma) {
     // the kernels store to output atom 3 F0 + i and read vectors[vector_index]:
     // every entry is checked before anything is uploaded
+    if (gamma->n_ops != G->g.G) return fail(BRILLE_ERR_INVALID_ARGUMENT, "GammaTable and point group sizes differ");
     const uint64_t m = (uint64_t)gamma->n_atoms * gamma->n_ops;
     if (m && (!gamma->F0 || !gamma->vector_index))
       return fail(BRILLE_ERR_INVALID_ARGUMENT, "GammaTable mappings must not be null");
@@ -320,12 +419,17 @@ int brille_grid_fill(brille_grid* G, const brille_data_desc* vals, const brille_
         return fail(BRILLE_ERR_INVALID_ARGUMENT, "GammaTable vector index out of range");
     }
   }
-  for (const brille_data_desc* x : {vals, vecs}) {
-    if (x->elements[1] % 3) return fail(BRILLE_ERR_INVALID_ARGUMENT, "Vectors must have 3N elements per branch");
-    if (x->elements[2] % 9) return fail(BRILLE_ERR_INVALID_ARGUMENT, "Matrices must have 9N elements per branch");
-    if (x->elements[0] + x->elements[1] + x->elements[2] == 0)
-      return fail(BRILLE_ERR_INVALID_ARGUMENT, "the elements per branch must not all be zero");
+  if (perm && perm->n_keys) {
+    if (!perm->keys || !perm->perm_index || (perm->n_perms && !perm->perms))
+      return fail(BRILLE_ERR_INVALID_ARGUMENT, "permutation table arrays must not be null");
+    for (uint64_t i = 0; i < perm->n_keys; ++i) {
+      if (perm->perm_index[i] >= perm->n_perms) return fail(BRILLE_ERR_INVALID_ARGUMENT, "permutation index out of range");
+      if (i && perm->keys[i] <= perm->keys[i - 1]) return fail(BRILLE_ERR_INVALID_ARGUMENT, "permutation keys must be sorted");
+    }
+    for (uint64_t i = 0; i < (uint64_t)perm->n_perms * vals->branches; ++i)
+      if (perm->perms[i] >= vals->branches) return fail(BRILLE_ERR_INVALID_ARGUMENT, "permutation entry out of range");
   }
+  (void)hipDeviceSynchronize();  // nothing still reads the payload that is replaced
   free_all(G->data_allocs);
   ++G->fill_gen;  // the payload tables are replaced from here on (batches re-read them)
   G->filled = false;
@@ -341,25 +445,12 @@ int brille_grid_fill(brille_grid* G, const brille_data_desc* vals, const brille_
   d.vec_rot = vecs->rotates_like;
   d.vec_lu = vecs->length_unit;
   auto& A = G->data_allocs;
-  int rc = BRILLE_OK;
-  auto chk = [&](int r) { if (r != BRILLE_OK && rc == BRILLE_OK) rc = r; };
   const uint32_t used = 2 * d.sc + d.sv;
   d.stride64 = row_stride(used, sizeof(double));
   d.stride32 = row_stride(used, sizeof(float));
-  {
-    std::vector<double> r64;
-    build_rows(d, vals->data, vecs->data, d.stride64, r64);
-    chk(upload(r64.data(), r64.size(), &d.rows64, A));
-  }
-  {
-    std::vector<double> r;
-    build_rows(d, vals->data, vecs->data, d.stride32, r);
-    std::vector<float> r32(r.size());
-    for (size_t i = 0; i < r.size(); ++i) r32[i] = static_cast<float>(r[i]);
-    chk(upload(r32.data(), r32.size(), &d.rows32, A));
-  }
+  int rc = fill_payload(G, vals, vecs, o);
+  auto chk = [&](int r) { if (r != BRILLE_OK && rc == BRILLE_OK) rc = r; };
   if (gamma) {
-    if (gamma->n_ops != G->g.G) rc = fail(BRILLE_ERR_INVALID_ARGUMENT, "GammaTable and point group sizes differ");
     d.n_atoms = gamma->n_atoms;
     d.gamma_ops = gamma->n_ops;
     chk(upload(gamma->F0, (size_t)gamma->n_atoms * gamma->n_ops, &d.F0, A));
@@ -368,12 +459,6 @@ int brille_grid_fill(brille_grid* G, const brille_data_desc* vals, const brille_
     if (gamma->cart_rotations) chk(upload(gamma->cart_rotations, 9 * (size_t)gamma->n_ops, &d.cart_rot, A));
   }
   if (perm && perm->n_keys) {
-    for (uint64_t i = 0; i < perm->n_keys; ++i) {
-      if (perm->perm_index[i] >= perm->n_perms) rc = fail(BRILLE_ERR_INVALID_ARGUMENT, "permutation index out of range");
-      if (i && perm->keys[i] <= perm->keys[i - 1]) rc = fail(BRILLE_ERR_INVALID_ARGUMENT, "permutation keys must be sorted");
-    }
-    for (uint64_t i = 0; i < (uint64_t)perm->n_perms * d.B; ++i)
-      if (perm->perms[i] >= d.B) rc = fail(BRILLE_ERR_INVALID_ARGUMENT, "permutation entry out of range");
     d.n_keys = perm->n_keys;
     chk(upload(perm->keys, (size_t)perm->n_keys, &d.keys, A));
     chk(upload(perm->perm_index, (size_t)perm->n_keys, &d.perm_index, A));
@@ -381,11 +466,29 @@ int brille_grid_fill(brille_grid* G, const brille_data_desc* vals, const brille_
   }
   if (rc != BRILLE_OK) {
     free_all(A);
+    std::memset(&d, 0, sizeof(d));
     G->filled = false;
     return rc;
   }
   G->filled = true;
   return BRILLE_OK;
+}
+
+int brille_grid_fill(brille_grid* G, const brille_data_desc* vals, const brille_data_desc* vecs,
+                     const brille_gamma_desc* gamma, const brille_perm_desc* perm) {
+  BRILLE_LOCK_GRID(G);
+  brille_fill_opts o{};
+  o.memory = BRILLE_MEM_HOST;
+  o.data_type = BRILLE_FP64;
+  o.keep = 0;
+  return grid_fill(G, vals, vecs, gamma, perm, o);
+}
+
+int brille_grid_fill_ex(brille_grid* G, const brille_data_desc* vals, const brille_data_desc* vecs,
+                        const brille_gamma_desc* gamma, const brille_perm_desc* perm, const brille_fill_opts* opts) {
+  BRILLE_LOCK_GRID(G);
+  if (!opts) return fail(BRILLE_ERR_INVALID_ARGUMENT, "null fill options");
+  return grid_fill(G, vals, vecs, gamma, perm, *opts);
 }
 
 uint64_t brille_bytes_per_point(const brille_grid* G, int32_t precision) {

@@ -83,6 +83,7 @@ int build_fast_tables(const brille_grid_desc* desc, brille_grid* G);
 void free_proposal(brille_grid* G);
 void free_staging(brille_grid* G);
 int check_rotation_modes(const brille_grid* G);
+int check_payload_precision(const brille_grid* G, int precision);
 int payload_layout(const DataConst& d);
 bool wide_rows(const DataConst& d);
 // binned pipeline (binned.hip)

@@ -498,5 +498,19 @@ PYBIND11_MODULE(_brille, m) {
       }, "subscript"_a)
       .def("total_node_volume", [](const PyGrid& g) { return g.g->trellis.total_node_volume(); })
       .def("tables", [](const PyGrid& g) { return grid_tables(*g.g); })
+      // the GammaTable (src/phonon.hpp:91-194) of the grid's lattice, as the
+      // tables() keys gamma_*: what a Gamma-rotating payload filled through
+      // the C ABI (brille_grid_fill_ex) needs, without a host payload
+      .def("gamma_tables", [](const PyGrid& pg) {
+        const BZTrellisQdc& g = *pg.g;
+        GammaTableHost gt;
+        gt.construct(g.bz.outer, g.bz.psym, g.bz.time_reversal, g.approx.direct, g.approx.digits);
+        py::dict t;
+        t["gamma_F0"] = vec_array(gt.F0, {(py::ssize_t)gt.n_atoms, (py::ssize_t)gt.n_ops});
+        t["gamma_vector_index"] = vec_array(gt.vindex, {(py::ssize_t)gt.n_atoms, (py::ssize_t)gt.n_ops});
+        t["gamma_vectors"] = vec_array(gt.vectors, {(py::ssize_t)(gt.vectors.size() / 3), 3});
+        t["gamma_cart_rotations"] = vec_array(gt.cart_rotations, {(py::ssize_t)gt.n_ops, 9});
+        return t;
+      })
       .def("device_handle", [](PyGrid& g) { return reinterpret_cast<uintptr_t>(g.g->handle()); });
 }

@@ -74,6 +74,15 @@ int check_rotation_modes(const brille_grid* G) {
   return BRILLE_OK;
 }
 
+// brille_grid_fill_ex may keep one precision of the payload only
+int check_payload_precision(const brille_grid* G, int precision) {
+  if (precision == BRILLE_FP64 && !G->d.rows64)
+    return fail(BRILLE_ERR_INVALID_ARGUMENT, "the payload was filled without its FP64 rows");
+  if (precision == BRILLE_FP32 && !G->d.rows32)
+    return fail(BRILLE_ERR_INVALID_ARGUMENT, "the payload was filled without its FP32 rows");
+  return BRILLE_OK;
+}
+
 using TileKernel = void (*)(GridConst, DataConst, LaunchArgs);
 
 // which specialised phase 2 the payload admits
@@ -193,6 +202,7 @@ int run_interpolate(brille_grid* G, const double* Q, uint64_t n, const brille_ru
     return fail(BRILLE_ERR_INVALID_ARGUMENT, "unknown precision");
   int rc = check_rotation_modes(G);
   if (rc != BRILLE_OK) return rc;
+  if (rc = check_payload_precision(G, opts.precision); rc != BRILLE_OK) return rc;
   if (n == 0) return BRILLE_OK;
   if (!Q || !values_out || !vectors_out) return fail(BRILLE_ERR_INVALID_ARGUMENT, "null input or output pointer");
   if (opts.use_proposal && !G->prop_trained)

@@ -114,6 +114,34 @@ def grid(cfg, volume_fraction=None, always_triangulate=False, with_vectors=None)
     return g
 
 
+def bare_grid(cfg, volume_fraction=None, always_triangulate=False):
+    """The BZTrellisQdc of a configuration, not filled (for payloads that are
+    generated and filled on the device: configuration 3's complex128
+    eigenvectors alone would be 83 GB on the host)."""
+    c = CONFIGS[cfg]
+    bz = _b.BrillouinZone(lattice(cfg))
+    vf = c["volume_fraction"] if volume_fraction is None else volume_fraction
+    return _b.BZTrellisQdc(bz, vf * bz.ir_polyhedron.volume, always_triangulate)
+
+
+def layout_tables(g, cfg):
+    """Tables of an unfilled grid plus its configuration's payload layout (scalar
+    eigenvalues, one Gamma-rotating 3-vector per atom in real-lattice units) and
+    GammaTable: what brille_grid_fill_ex needs besides the payload itself."""
+    c = CONFIGS[cfg]
+    n_atoms = c["modes"] // 3
+    t = dict(g.tables())
+    t.update(g.gamma_tables())
+    t["values_elements"] = [1, 0, 0]
+    t["vectors_elements"] = [0, 3 * n_atoms, 0]
+    t["values_rotates_like"], t["values_length_unit"] = 0, 0
+    t["vectors_rotates_like"], t["vectors_length_unit"] = 2, 3
+    t["branches"] = c["modes"]
+    for k in ("values", "vectors"):
+        t.pop(k, None)
+    return t
+
+
 def q_points(cfg, n=None, stream=2):
     c = CONFIGS[cfg]
     n = c["n_q"] if n is None else n

@@ -184,11 +184,11 @@ typedef struct brille_perm_desc {
 
 enum brille_precision { BRILLE_FP64 = 0, BRILLE_FP32 = 1 };
 enum brille_memory { BRILLE_MEM_DEVICE = 0, BRILLE_MEM_HOST = 1 };
-/* AUTO and TUNED pick TILE or BINNED by size; PLAIN is the literal FP64
- * restatement (exact fold-in, atan2 antiphases); TILE is the fused one-launch
- * kernel; BINNED buckets points by trellis node range first so the payload
- * gathers hit in cache (device work space, not used when aux outputs are
- * requested: those calls run TILE). */
+/* AUTO and TUNED pick TILE or BINNED by size (TILE when any aux output is
+ * requested); PLAIN is the literal FP64 restatement (exact fold-in, atan2
+ * antiphases); TILE is the fused one-launch kernel; BINNED buckets points by
+ * trellis node range first so the payload gathers hit in cache (device work
+ * space; reports every aux output but tau). */
 enum brille_kernel {
   BRILLE_KERNEL_AUTO = 0,
   BRILLE_KERNEL_PLAIN = 1,
@@ -234,6 +234,23 @@ int brille_grid_fill(brille_grid* grid, const brille_data_desc* values,
                      const brille_data_desc* vectors, const brille_gamma_desc* gamma,
                      const brille_perm_desc* perm);
 int brille_grid_destroy(brille_grid* grid);
+/* brille_grid_fill for payloads that are not host float64 / complex128 (no
+ * reference counterpart: the reference's fill takes float64 and complex128
+ * arrays, wrap/_common_grid.hpp:26-124).  values->data and vectors->data point
+ * to float / complex float when data_type is BRILLE_FP32 and may live in device
+ * memory; only the precisions named in `keep` are packed (a call at another
+ * precision then fails), so a payload of tens of GB needs no host copy and no
+ * second device copy.  Checks run before the previous payload is released: a
+ * failed fill leaves it in place unless packing itself fails. */
+enum brille_keep { BRILLE_KEEP_BOTH = 0, BRILLE_KEEP_FP64 = 1, BRILLE_KEEP_FP32 = 2 };
+typedef struct brille_fill_opts {
+  int32_t memory;     /* brille_memory of values->data and vectors->data */
+  int32_t data_type;  /* BRILLE_FP64: double / complex double; BRILLE_FP32: float / complex float */
+  int32_t keep;       /* brille_keep: which payload precisions to pack */
+} brille_fill_opts;
+int brille_grid_fill_ex(brille_grid* grid, const brille_data_desc* values, const brille_data_desc* vectors,
+                        const brille_gamma_desc* gamma, const brille_perm_desc* perm,
+                        const brille_fill_opts* opts);
 /* bytes of interpolation output per Q at a precision  src/interpolatordual.hpp:297-299 */
 uint64_t brille_bytes_per_point(const brille_grid* grid, int32_t precision);
 

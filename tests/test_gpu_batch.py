@@ -147,7 +147,7 @@ def test_batch_reports_failing_grid(small_batch):
 def test_batch_follows_a_refilled_member(small_batch):
     """brille_grid_fill replaces a member's payload buffers: the next batch run describes that
     grid again (its fill generation changed) and gives what a per-grid call on the new payload
-    gives; an emptied member stops the run instead of reading freed memory."""
+    gives.  A fill that fails its checks leaves the previous payload in place."""
     grids, tables, devs, Q, offsets = small_batch
     say("batch after a member is refilled")
     t0 = dict(tables[0])
@@ -169,10 +169,12 @@ def test_batch_follows_a_refilled_member(small_batch):
     np.testing.assert_array_equal(vals[0].to_host(), v)
     np.testing.assert_array_equal(vecs[0].to_host(), c)
     assert not np.array_equal(before, v)
-    # a failed fill leaves the member unfilled: the batch reports it
-    bad = dict(t0)
-    bad["values"] = np.asarray(t0["values"])[:10]
-    with pytest.raises(capi.BrilleError):
+    # a fill that fails its checks changes nothing
+    bad = dict(t1)
+    bad["gamma_F0"] = np.asarray(t1["gamma_F0"]).copy()
+    bad["gamma_F0"].flat[0] = 99
+    with pytest.raises(capi.BrilleError, match="GammaTable"):
         d0.fill(bad)
-    rc = batch.run(dQ, off, vals, vecs, precision=capi.FP32)
-    assert capi.STATUS_NAMES[rc] == "NOT_FILLED"
+    assert batch.run(dQ, off, vals, vecs, precision=capi.FP32) == 0
+    np.testing.assert_array_equal(vals[0].to_host(), v)
+    np.testing.assert_array_equal(vecs[0].to_host(), c)
