@@ -134,7 +134,7 @@ def layout_tables(g, cfg):
     t.update(g.gamma_tables())
     t["values_elements"] = [1, 0, 0]
     t["vectors_elements"] = [0, 3 * n_atoms, 0]
-    t["values_rotates_like"], t["values_length_unit"] = 0, 0
+    t["values_rotates_like"], t["values_length_unit"] = 0, 3  # the defaults fill gives scalars
     t["vectors_rotates_like"], t["vectors_length_unit"] = 2, 3
     t["branches"] = c["modes"]
     for k in ("values", "vectors"):

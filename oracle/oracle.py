@@ -103,6 +103,19 @@ def moveinto(tables, Q):
     return rc, q, tau, st
 
 
+def get_xyz(tables, q):
+    """Cartesian coordinates of rlu rows q as the reference forms them (LVec::get_xyz,
+    src/array_lvec_methods.tpp:40-51, via utils::multiply_matrix_vector,
+    src/utilities.tpp:44-55): x_i = (B_i0 q_0 + B_i1 q_1) + B_i2 q_2, each product
+    and sum rounded on its own (numpy ufuncs; a matmul may sum in another order)."""
+    B = np.asarray(tables["outer_to_xyz"], dtype=np.float64).reshape(3, 3)
+    q = _d(q).reshape(-1, 3)
+    x = np.empty_like(q)
+    for i in range(3):
+        x[:, i] = (B[i, 0] * q[:, 0] + B[i, 1] * q[:, 1]) + B[i, 2] * q[:, 2]
+    return x
+
+
 def locate(tables, xyz):
     th = TableHolder(tables)
     x = _d(xyz).reshape(-1, 3)

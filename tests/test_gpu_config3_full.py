@@ -2,9 +2,9 @@
 
 Monoclinic P2_1/c, 24 atoms, 72 modes, about 1.07e6 trellis vertices: the
 FP32 payload (72 x 72 complex64 eigenvectors per vertex) is 42.6 GB, and the
-complex128 one (83 GB) would not fit the host, so the payload is generated on
-the device (torch, as plumbing) and handed to brille_grid_fill_ex as FP32
-device arrays, keeping the FP32 rows only.  At this size the wide-row
+complex128 one (83 GB) is not formed at all: the payload is generated as
+complex64 and handed to brille_grid_fill_ex as FP32 data, keeping the FP32
+rows only.  At this size the wide-row
 antiphase table (8 per cube, 4 per tetrahedron, per mode) exceeds the 4 GiB
 the sums address, so every antiphase is computed on the fly.
 
@@ -38,7 +38,6 @@ def say(*a):
 
 @pytest.fixture(scope="module")
 def cfg3():
-    import torch
     g = S.bare_grid(3)
     t = S.layout_tables(g, 3)
     V = t["vertices"].shape[0]
@@ -50,22 +49,19 @@ def cfg3():
     gk = r.normal(size=(B, 3 * A, 3)) * 0.05
     u = r.normal(size=(B, 3 * A)) + 1j * r.normal(size=(B, 3 * A))
     u /= np.linalg.norm(u, axis=1, keepdims=True)
-    dev_t = torch.device("cuda:0")
-    x = torch.as_tensor(np.asarray(t["vertices"]), device=dev_t)
-    vals = (x @ torch.as_tensor(a.T, device=dev_t) + torch.as_tensor(c, device=dev_t)).float().contiguous()
-    vecs = torch.empty((V, B, 3 * A), dtype=torch.complex64, device=dev_t)
-    gk_t = torch.as_tensor(gk.reshape(-1, 3).T.copy(), device=dev_t)
-    u_t = torch.as_tensor(u, device=dev_t)
-    step = 65536
+    x = np.asarray(t["vertices"])
+    vals = (x @ a.T + c).astype(np.float32)
+    vecs = np.empty((V, B, 3 * A), np.complex64)
+    gk2 = gk.reshape(-1, 3).T.astype(np.float32)
+    u32 = u.astype(np.complex64)
+    step = 32768
     for s0 in range(0, V, step):
-        amp = 1.0 + (x[s0:s0 + step] @ gk_t).reshape(-1, B, 3 * A)
-        vecs[s0:s0 + step] = (amp * u_t).to(torch.complex64)
-    torch.cuda.synchronize()
-    say(f"payload generated on the device: {vecs.numel() * 8 / 1e9:.1f} GB of complex64")
+        amp = 1.0 + (x[s0:s0 + step].astype(np.float32) @ gk2).reshape(-1, B, 3 * A)
+        np.multiply(amp, u32, out=vecs[s0:s0 + step])
+    say(f"payload generated: {vecs.nbytes / 1e9:.1f} GB of complex64 on the host")
     dev = capi.Grid(t)
-    dev.fill_ex(t, vals.data_ptr(), vecs.data_ptr(), capi.MEM_DEVICE, capi.FP32, capi.KEEP_FP32)
-    del vals, vecs, x
-    torch.cuda.empty_cache()
+    dev.fill_ex(t, vals.ctypes.data, vecs.ctypes.data, capi.MEM_HOST, capi.FP32, capi.KEEP_FP32)
+    del vals, vecs
     say("filled (FP32 rows only)")
     fields = dict(a=a, c=c, gk=gk, u=u)
     return g, t, dev, fields
@@ -83,8 +79,7 @@ def test_indices_match_the_oracle_and_binned_equals_tile(cfg3):
     say("config 3 full: oracle fold-in and location on 1e5 Q")
     rc, q, tau, rot, invrot, st = O.ir_moveinto(t, Q)
     assert rc == 0
-    B = np.asarray(t["outer_to_xyz"]).reshape(3, 3)
-    rc, node, tet, niw, idx, w = O.locate(t, q @ B.T)
+    rc, node, tet, niw, idx, w = O.locate(t, O.get_xyz(t, q))
     assert rc == 0
     want = dict(rot=rot, invrot=invrot, node=node, tet=tet, n_iw=niw, iw_index=idx, iw_weight=w, status=st)
     say("config 3 full: TILE FP32")

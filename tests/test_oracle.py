@@ -284,3 +284,19 @@ def test_earlier_stage_errors_win_over_earlier_indices():
     rc, v, c, aux, bad2 = O.ir_interpolate_at(t, Q, first_zone=True)
     assert aux["status"][0] == 3 and aux["status"][51] == 1
     assert rc == 3 and bad2 == 51  # BRILLE_ERR_OUTSIDE_BZ at the NaN, not the null node at 0
+
+
+def test_table_only_locate_equals_the_full_path():
+    """ir_moveinto + get_xyz + locate (tables only, no payload) give the full path's q, tau, Ridx,
+    node, tetrahedron and weights bit for bit: how a payload too large for the host is checked."""
+    from brille_amd import synthetic as S
+    g = S.grid(1)
+    t = g.tables()
+    Q = S.q_points(1, 20000, stream=44)
+    rc, v, c, aux, _ = O.ir_interpolate_at(t, Q, nthreads=8)
+    rc1, q, tau, r, ir, st = O.ir_moveinto(t, Q)
+    rc2, node, tet, niw, idx, w = O.locate(t, O.get_xyz(t, q))
+    assert rc == rc1 == rc2 == 0
+    for a, b in ((q, aux["ir_q"]), (tau, aux["tau"]), (r, aux["rot"]), (node, aux["node"]), (tet, aux["tet"]),
+                 (niw, aux["n_iw"]), (idx, aux["iw_index"]), (w, aux["iw_weight"])):
+        np.testing.assert_array_equal(a, b)
