@@ -114,6 +114,8 @@ SIGNATURES = {
                                            C.c_void_p, C.POINTER(AuxOut), C.c_void_p]),
     "brille_interpolate_at": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(RunOpts), C.c_void_p,
                                         C.c_void_p, C.POINTER(AuxOut), C.c_void_p]),
+    "brille_ir_interpolate_at_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(RunOpts),
+                                                   C.c_void_p, C.c_void_p, C.POINTER(AuxOut), C.c_void_p]),
     "brille_ir_moveinto": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_int32, C.c_void_p, C.c_void_p,
                                      C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "brille_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_uint32, C.POINTER(C.c_void_p)]),

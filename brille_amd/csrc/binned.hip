@@ -765,22 +765,48 @@ struct PassBuffers {
 // scatter, node order) and sums over all passes, each from events on the
 // stream that ran it (with several passes the stages overlap, so they add up
 // to more than the call)
+// Byte offsets of one pass's buffers (a half of the work space) for passes of
+// `chunk` points.
+struct PassLayout {
+  uint64_t rec, srt, hist, part, list, hint, perm, node, cnt, half;
+  uint64_t scan_blocks;  // segments of the histogram scan (k_scan_top handles at most 256 Ki)
+};
+static PassLayout pass_layout(uint64_t chunk, uint32_t nb) {
+  const uint32_t nblk_max = (uint32_t)((chunk + BIN_SPAN - 1) / BIN_SPAN);
+  const uint64_t m_max = (uint64_t)(nb + 1) * nblk_max;
+  const uint64_t np_max = (m_max + SCAN_SEG - 1) / SCAN_SEG;
+  auto al = [](uint64_t b) { return (b + 255) & ~uint64_t(255); };
+  PassLayout L;
+  L.rec = 0;
+  L.srt = al(chunk * sizeof(FoldRec));
+  L.hist = L.srt + al(chunk * sizeof(SortRec));
+  L.part = L.hist + al(m_max * 4);
+  L.list = L.part + al(np_max * 4);
+  L.hint = L.list + al(chunk * 4);
+  L.perm = L.hint + al(chunk * 4);
+  L.node = L.perm + al(chunk * 4);
+  L.cnt = L.node + al(chunk * 4);
+  L.half = L.cnt + 256;
+  L.scan_blocks = np_max;
+  return L;
+}
+
+uint64_t binned_pass_bytes(const brille_grid* G, uint64_t chunk) { return pass_layout(chunk, bucket_count(G)).half; }
+
+// The largest pass (a power of two, at most BIN_CHUNK points) whose buffers fit `bytes`.
+uint64_t binned_pass_for(const brille_grid* G, uint64_t bytes) {
+  uint64_t c = BIN_CHUNK;
+  while (c > BIN_SPAN && binned_pass_bytes(G, c) > bytes) c >>= 1;
+  return binned_pass_bytes(G, c) <= bytes ? c : 0;
+}
+
 int launch_binned(brille_grid* G, const LaunchArgs& a, int precision, hipStream_t s, float* stage_ms) {
   if (int rc = ensure_cube_rows(G, precision, s); rc != BRILLE_OK) return rc;
   const uint64_t chunk = std::min<uint64_t>(a.n, BIN_CHUNK);
   if (chunk == 0) return BRILLE_OK;
   const uint64_t passes = (a.n + chunk - 1) / chunk;
   const int halves = passes > 1 ? 2 : 1;
-  const uint32_t nb = bucket_count(G);
-  const uint32_t nblk_max = (uint32_t)((chunk + BIN_SPAN - 1) / BIN_SPAN);
-  const uint64_t m_max = (uint64_t)(nb + 1) * nblk_max;
-  const uint64_t np_max = (m_max + SCAN_SEG - 1) / SCAN_SEG;
-  auto al = [](uint64_t b) { return (b + 255) & ~uint64_t(255); };
-  const uint64_t o_rec = 0, o_srt = al(chunk * sizeof(FoldRec)), o_hist = o_srt + al(chunk * sizeof(SortRec)),
-                 o_part = o_hist + al(m_max * 4), o_list = o_part + al(np_max * 4), o_hint = o_list + al(chunk * 4),
-                 o_perm = o_hint + al(chunk * 4), o_node = o_perm + al(chunk * 4), o_cnt = o_node + al(chunk * 4),
-                 half = o_cnt + 256, need = half * (uint64_t)halves;
-  if (np_max > 256ull * 1024) return fail(BRILLE_ERR_INVALID_ARGUMENT, "binned pass too large");
+  const uint64_t need = binned_pass_bytes(G, chunk) * (uint64_t)halves;
   if (G->bin_ws_bytes < need) {
     free_work_space(G);
     if (hipMalloc(&G->bin_ws, need) != hipSuccess) return fail(BRILLE_ERR_DEVICE, "hipMalloc failed for the binned work space");
@@ -788,10 +814,25 @@ int launch_binned(brille_grid* G, const LaunchArgs& a, int precision, hipStream_
   }
   if (halves > 1)
     if (int rc = ensure_side_stream(G); rc != BRILLE_OK) return rc;
+  return run_binned_passes(G, a, precision, s, stage_ms, G->bin_ws, chunk, halves);
+}
+
+// The pipeline over work space `ws` in passes of `chunk` points; with two
+// halves the preparation of the next pass overlaps the sums (side stream
+// and events of the grid), with one everything runs on `s` in order.
+int run_binned_passes(brille_grid* G, const LaunchArgs& a, int precision, hipStream_t s, float* stage_ms, void* ws_base,
+                      uint64_t chunk, int halves) {
+  if (a.n == 0) return BRILLE_OK;
+  const uint64_t passes = (a.n + chunk - 1) / chunk;
+  const uint32_t nb = bucket_count(G);
+  const PassLayout Lo = pass_layout(chunk, nb);
+  if (Lo.scan_blocks > 256ull * 1024) return fail(BRILLE_ERR_INVALID_ARGUMENT, "binned pass too large");
+  const uint64_t o_rec = Lo.rec, o_srt = Lo.srt, o_hist = Lo.hist, o_part = Lo.part, o_list = Lo.list,
+                 o_hint = Lo.hint, o_perm = Lo.perm, o_node = Lo.node, o_cnt = Lo.cnt, half = Lo.half;
   const bool prop = a.use_proposal != 0;
   PassBuffers buf[2];
   for (int h = 0; h < halves; ++h) {
-    char* ws = static_cast<char*>(G->bin_ws) + h * half;
+    char* ws = static_cast<char*>(ws_base) + h * half;
     PassBuffers& b = buf[h];
     b.rec = reinterpret_cast<FoldRec*>(ws + o_rec);
     b.srt = reinterpret_cast<SortRec*>(ws + o_srt);

@@ -84,11 +84,20 @@ void free_proposal(brille_grid* G);
 void free_staging(brille_grid* G);
 int check_rotation_modes(const brille_grid* G);
 int check_payload_precision(const brille_grid* G, int precision);
+// the fused tile kernel of one call (fast: tuned fold-in and sums; else the literal restatement)
+void launch_tile(const brille_grid* G, const LaunchArgs& a, int precision, bool fast, hipStream_t s);
 int payload_layout(const DataConst& d);
 bool wide_rows(const DataConst& d);
 // binned pipeline (binned.hip)
 bool binned_applies(const brille_grid* G, const LaunchArgs& a, int kernel);
 int launch_binned(brille_grid* G, const LaunchArgs& a, int precision, hipStream_t s, float* stage_ms);
+// the pipeline on a given work space: passes of `chunk` points, `halves` = 2
+// overlaps preparation and sums (grid side stream), 1 runs all on `s`
+int run_binned_passes(brille_grid* G, const LaunchArgs& a, int precision, hipStream_t s, float* stage_ms, void* ws,
+                      uint64_t chunk, int halves);
+// bytes of one pass's buffers, and the largest pass whose buffers fit `bytes` (0: none)
+uint64_t binned_pass_bytes(const brille_grid* G, uint64_t chunk);
+uint64_t binned_pass_for(const brille_grid* G, uint64_t bytes);
 void free_binned(brille_grid* G);
 // clears n 32-bit words on stream s with a kernel (no memset node, so that a
 // caller may capture the call in a graph)

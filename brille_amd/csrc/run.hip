@@ -141,6 +141,10 @@ static void launch_sel(const brille_grid* G, const LaunchArgs& a, int precision,
   hipLaunchKernelGGL(k, dim3((unsigned)blocks), dim3(TILE), 0, s, G->g, G->d, a);
 }
 
+void launch_tile(const brille_grid* G, const LaunchArgs& a, int precision, bool fast, hipStream_t s) {
+  launch_sel(G, a, precision, fast, s);
+}
+
 // every kernel of one call on device buffers
 static int dispatch(brille_grid* G, const LaunchArgs& a, const brille_run_opts& o, hipStream_t s) {
   const bool fast = o.kernel != BRILLE_KERNEL_PLAIN;

@@ -267,6 +267,20 @@ int brille_ir_interpolate_at(brille_grid* grid, const double* Q, uint64_t n,
 int brille_interpolate_at(brille_grid* grid, const double* Q, uint64_t n,
                           const brille_run_opts* opts, void* values_out,
                           void* vectors_out, const brille_aux_out* aux, void* stream);
+/* No counterpart in the reference.  brille_ir_interpolate_at for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc): it allocates nothing.  Its error
+ * record and the binned pipeline's work space live in one static device array
+ * of 64 MiB per GPU (held from module load until the process ends); passes
+ * are sized to fit it (2^19 points) and run on `stream` alone, and the row
+ * table is used only if an earlier call built it.  Q, the outputs and aux must
+ * be device memory (opts->memory = BRILLE_MEM_DEVICE); everything else is as
+ * brille_ir_interpolate_at, results bit for bit the same.  Calls share their
+ * GPU's array: calls whose work overlaps on one GPU race (use one stream, or
+ * synchronise between them). */
+int brille_ir_interpolate_at_noalloc(brille_grid* grid, const double* Q, uint64_t n,
+                                     const brille_run_opts* opts, void* values_out, void* vectors_out,
+                                     const brille_aux_out* aux, void* stream);
 /* stage (a) alone: q, tau, Ridx, invRidx  src/bz_move.cpp:165-296 */
 int brille_ir_moveinto(brille_grid* grid, const double* Q, uint64_t n, int32_t memory,
                        double* q_out, int32_t* tau_out, uint32_t* rot_out,
