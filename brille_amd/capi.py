@@ -326,6 +326,15 @@ class DeviceArray:
             check(lib().brille_memcpy_d2h(out.ctypes.data_as(C.c_void_p), self.ptr, self.nbytes))
         return out
 
+    def rows_to_host(self, r0, r1):
+        """Rows [r0, r1) of the leading axis, copied to the host."""
+        row = self.nbytes // max(self.shape[0], 1)
+        out = np.empty((r1 - r0,) + self.shape[1:], self.dtype)
+        if out.nbytes:
+            check(lib().brille_memcpy_d2h(out.ctypes.data_as(C.c_void_p), C.c_void_p(self.ptr.value + r0 * row),
+                                          out.nbytes))
+        return out
+
     def free(self):
         if self.ptr:
             lib().brille_device_free(self.ptr)

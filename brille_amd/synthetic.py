@@ -204,55 +204,91 @@ def _batch_basis(sg, target, r):
     return np.array(pos), np.array(typ)
 
 
-def batch_grids(n_grids=64, v_range=(2e3, 2e5), n_q=10_000_000, box=3.0, cfg=2, verbose=None):
-    """Configuration 2: n_grids filled BZTrellisQdc of uneven vertex and mode
-    counts and their Q.  Returns (grids, Q, offsets): grid g owns
-    Q[offsets[g]:offsets[g+1]].
-
-    Per grid: space group cycling BATCH_SPACEGROUPS, lattice lengths U(3,8) A
-    (monoclinic beta U(95,115) deg), a symmetry-closed basis of 1-24 atoms
-    (target log-uniform), 3A modes, a node volume solved (one trial build)
-    for a vertex count log-uniform in v_range, capped so the complex128
-    eigenvectors stay under BATCH_PAYLOAD_CAP bytes.  Payload: smooth seeded
-    eigenvalues and random unit eigenvectors (not symmetrised: the batch
-    exercises scheduling over uneven grids, not the physics).  Q: n_q points
-    uniform in [-box, box]^3 split across grids with Dirichlet(0.3) weights."""
+def batch_specs(n_grids=64, v_range=(2e3, 2e5), cfg=2, payload_cap=BATCH_PAYLOAD_CAP):
+    """Per-grid parameters of configuration 2: space group cycling
+    BATCH_SPACEGROUPS, lattice lengths U(3,8) A (monoclinic beta U(95,115)
+    deg), a symmetry-closed basis of 1-24 atoms (target log-uniform), 3A modes
+    and a vertex count log-uniform in v_range -- optionally capped so that the
+    complex128 eigenvectors stay under payload_cap bytes (None: no cap)."""
     r = rng(cfg, 0)
-    grids = []
+    specs = []
     for gi in range(n_grids):
         sg = BATCH_SPACEGROUPS[gi % len(BATCH_SPACEGROUPS)]
         lengths, angles = _batch_lattice_parameters(sg, r)
         target_atoms = int(round(np.exp(r.uniform(0.0, np.log(24.0)))))
         pos, typ = _batch_basis(sg, target_atoms, r)
         A = len(pos)
-        B = 3 * A
         v_target = float(np.exp(r.uniform(np.log(v_range[0]), np.log(v_range[1]))))
-        v_target = min(v_target, BATCH_PAYLOAD_CAP / (B * 3 * A * 16.0))
+        if payload_cap is not None:
+            v_target = min(v_target, payload_cap / (3 * A * 3 * A * 16.0))
         v_target = max(v_target, min(v_range[0], 200.0))
-        lat = _b.Direct(np.array(lengths), np.array(angles), sg, _b.Basis(pos, typ))
-        bz = _b.BrillouinZone(lat)
-        vir = bz.ir_polyhedron.volume
-        trial_nv = vir / 300.0
-        trial = _b.BZTrellisQdc(bz, trial_nv)
-        nv = trial_nv * (trial.rlu.shape[0] / v_target) ** 1.0
-        g = _b.BZTrellisQdc(bz, nv)
-        rlu = g.rlu
-        V = rlu.shape[0]
-        rv = rng(cfg, 100 + gi)
-        Gk = rv.integers(-2, 3, size=(B, 2, 3)).astype(np.float64)
-        Ak = rv.uniform(0.5, 5.0, size=(B, 2))
-        vals = np.empty((V, B, 1))
-        for j in range(B):
-            vals[:, j, 0] = np.cos(2 * np.pi * rlu @ Gk[j].T) @ Ak[j]
-        vecs = rv.normal(size=(V, B, A, 3)) + 1j * rv.normal(size=(V, B, A, 3))
-        vecs /= np.linalg.norm(vecs.reshape(V, B, -1), axis=2)[:, :, None, None]
-        g.fill(vals, np.array([1, 0, 0]), vecs, np.array([0, 3 * A, 0, 2, 3]))
-        grids.append(g)
-        if verbose:
-            verbose(f"grid {gi}: sg {sg} A={A} B={B} V={V} (target {v_target:.0f})")
+        specs.append(dict(index=gi, sg=sg, lengths=lengths, angles=angles, pos=pos, typ=typ, atoms=A,
+                          modes=3 * A, v_target=v_target))
+    return specs
+
+
+def batch_grid(spec):
+    """The unfilled BZTrellisQdc of one batch grid: the node volume is solved
+    from one trial build for the spec's vertex count."""
+    lat = _b.Direct(np.array(spec["lengths"]), np.array(spec["angles"]), spec["sg"],
+                    _b.Basis(spec["pos"], spec["typ"]))
+    bz = _b.BrillouinZone(lat)
+    trial_nv = bz.ir_polyhedron.volume / 300.0
+    trial = _b.BZTrellisQdc(bz, trial_nv)
+    return _b.BZTrellisQdc(bz, trial_nv * trial.rlu.shape[0] / spec["v_target"])
+
+
+def batch_payload(g, spec, cfg=2):
+    """Payload of one batch grid: smooth seeded eigenvalues (V, B, 1) and random
+    unit eigenvectors (V, B, A, 3) complex128, each mode's scattered about a
+    direction of its own (u_b + 0.3 noise, normalised), so that overlaps
+    between vertices stay far from zero and their antiphases are well
+    conditioned in FP32 (not symmetrised: the batch exercises scheduling over
+    uneven grids, not the physics)."""
+    rlu = g.rlu
+    V, B, A = rlu.shape[0], spec["modes"], spec["atoms"]
+    rv = rng(cfg, 100 + spec["index"])
+    Gk = rv.integers(-2, 3, size=(B, 2, 3)).astype(np.float64)
+    Ak = rv.uniform(0.5, 5.0, size=(B, 2))
+    vals = np.empty((V, B, 1))
+    for j in range(B):
+        vals[:, j, 0] = np.cos(2 * np.pi * rlu @ Gk[j].T) @ Ak[j]
+    u = rv.normal(size=(B, A, 3)) + 1j * rv.normal(size=(B, A, 3))
+    u /= np.linalg.norm(u.reshape(B, -1), axis=1)[:, None, None]
+    vecs = rv.normal(size=(V, B, A, 3)) + 1j * rv.normal(size=(V, B, A, 3))
+    vecs *= 0.3 / np.sqrt(2 * 3 * A)
+    vecs += u[None]
+    vecs /= np.linalg.norm(vecs.reshape(V, B, -1), axis=2)[:, :, None, None]
+    return vals, vecs
+
+
+def batch_queries(n_grids=64, n_q=10_000_000, box=3.0, cfg=2):
+    """Q of configuration 2: n_q points uniform in [-box, box]^3 split across
+    grids with Dirichlet(0.3) weights; grid g owns Q[offsets[g]:offsets[g+1]]."""
     w = rng(cfg, 1).dirichlet(np.full(n_grids, 0.3))
     counts = np.floor(w * n_q).astype(np.int64)
     counts[np.argmax(counts)] += int(n_q) - int(counts.sum())
     offsets = np.concatenate([[0], np.cumsum(counts)]).astype(np.uint64)
     Q = rng(cfg, 2).uniform(-box, box, size=(int(n_q), 3))
+    return Q, offsets
+
+
+def batch_grids(n_grids=64, v_range=(2e3, 2e5), n_q=10_000_000, box=3.0, cfg=2, verbose=None,
+                payload_cap=BATCH_PAYLOAD_CAP):
+    """Configuration 2 held on the host: n_grids filled BZTrellisQdc of uneven
+    vertex and mode counts and their Q (batch_specs, batch_grid,
+    batch_payload, batch_queries).  Returns (grids, Q, offsets).  The default
+    cap keeps every grid's complex128 payload on the host small; the full
+    configuration streams grid by grid instead (payload_cap=None)."""
+    grids = []
+    for spec in batch_specs(n_grids, v_range, cfg, payload_cap):
+        g = batch_grid(spec)
+        vals, vecs = batch_payload(g, spec, cfg)
+        A = spec["atoms"]
+        g.fill(vals, np.array([1, 0, 0]), vecs, np.array([0, 3 * A, 0, 2, 3]))
+        grids.append(g)
+        if verbose:
+            verbose(f"grid {spec['index']}: sg {spec['sg']} A={A} B={3 * A} V={g.rlu.shape[0]} "
+                    f"(target {spec['v_target']:.0f})")
+    Q, offsets = batch_queries(n_grids, n_q, box, cfg)
     return grids, Q, offsets
