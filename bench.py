@@ -218,6 +218,11 @@ def main():
         "cpu_baseline": cpu,
         "plain": plain_res,
         "speedup_vs_plain": (plain_res["ms_per_step"] / wall_fast) if plain_res else None,
+        # the same comparison against the strongest baselines, not only the literal FP64 kernel
+        "speedup_vs_binned_no_proposal": (results["binned fp32"]["ms_per_step"] / wall_fast
+                                          if head_name != "binned fp32" else 1.0),
+        "speedup_vs_tile_fp32": results["tile fp32"]["ms_per_step"] / wall_fast,
+        "speedup_vs_binned_fp64": results["binned fp64"]["ms_per_step"] / wall_fast,
         "paths": results,
         "learned": learned,
         "fp32_error": fp32_error,
@@ -326,7 +331,8 @@ def bench_batch(args, capi, S):
 def batch_cpu_baseline(tables, Q, offsets, seconds):
     """The oracle on a bounded sample: the same fraction of every grid's points."""
     from oracle import oracle as O
-    cores = min(16, os.cpu_count() or 1)
+    hw = host_cpu()
+    cores = hw["threads"]
     counts = np.diff(offsets.astype(np.int64))
     frac = 2e-3
     total, tsum = 0, 0.0
@@ -344,7 +350,7 @@ def batch_cpu_baseline(tables, Q, offsets, seconds):
     log(f"cpu baseline: {total / tsum:.4g} Q/s on {cores} threads ({total} Q)")
     return dict(value=total / tsum, unit="Q-points/s", cores=cores, kind="port",
                 sample=f"{total} Q: the first {frac:g} of each grid's points (grids in order until {seconds:g} s), "
-                       f"FP64, {cores} OpenMP threads")
+                       f"FP64, {cores} OpenMP threads", cpu=hw, reference_published=REFERENCE_PUBLISHED)
 
 
 def measured_traffic(cfg, path_name):
@@ -362,11 +368,42 @@ def measured_traffic(cfg, path_name):
     return None
 
 
+def host_cpu():
+    """Host CPU model, the thread counts the process sees and the threads used:
+    16 at most, the host-thread share a one-GPU job is given on the GPU box
+    (more may be visible there, but they serve other jobs)."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return dict(model=model, nproc=os.cpu_count(), usable=usable,
+                omp_num_threads=os.environ.get("OMP_NUM_THREADS"), threads=min(16, usable))
+
+
+# the reference's one published measurement of this path: 1e5 Q in 19.56 s on
+# 6 OpenMP threads (stage (a) 7.14 s, (b)+(c1) 9.13 s, (c2) 3.29 s), La2Zr2O7
+# grid, brille 0.4.3, unspecified CPU (validation/profiling.md:85-93)
+REFERENCE_PUBLISHED = dict(value=1e5 / 19.56, unit="Q-points/s", threads=6,
+                           stages_s={"(a) ir_moveinto": 7.14, "(b)+(c1) locate + sums": 9.13, "(c2) rotation": 3.29},
+                           source="validation/profiling.md:85-93 (1e5 Q, La2Zr2O7 grid, 6 threads)")
+
+
 def cpu_baseline(T, cfg, c, seconds):
-    """The oracle (C, OpenMP, the reference's loop structure) on a bounded sample."""
+    """The oracle (C, OpenMP, the reference's loop structure: serial fold-in,
+    dynamic-scheduled location and sums, static-scheduled rotation) on a
+    bounded sample of the same workload, with its stage times."""
     from brille_amd import synthetic as S
     from oracle import oracle as O
-    cores = min(16, os.cpu_count() or 1)
+    hw = host_cpu()
+    cores = hw["threads"]
     probe = S.q_points(cfg, 20000, stream=7)
     tp = time.perf_counter()
     O.ir_interpolate_at(T, probe, nthreads=cores)
@@ -376,9 +413,18 @@ def cpu_baseline(T, cfg, c, seconds):
     tc = time.perf_counter()
     rc, *_ = O.ir_interpolate_at(T, sample, nthreads=cores)
     cpu_s = time.perf_counter() - tc
-    log(f"cpu baseline: {m / cpu_s:.4g} Q/s on {cores} threads")
+    st = O.stage_seconds()
+    log(f"cpu baseline: {m / cpu_s:.4g} Q/s on {cores} threads, stages {st}")
     return dict(value=m / cpu_s, unit="Q-points/s", cores=cores, kind="port",
-                sample=f"{m} Q of the same workload (uniform in [-{c['box']},{c['box']}]^3), FP64, {cores} OpenMP threads")
+                sample=f"{m} Q of the same workload (uniform in [-{c['box']},{c['box']}]^3), FP64, {cores} OpenMP threads",
+                cpu=hw,
+                stages_s={"(a) ir_moveinto (serial)": round(float(st[0]), 4),
+                          "(b)+(c1) locate + sums": round(float(st[1]), 4),
+                          "(c2) rotation": round(float(st[2]), 4)},
+                note="the timed code is this build's C restatement of the reference's loops (the reference needs "
+                     "HighFive to compile); it is much faster than the reference's one published figure, "
+                     "given beside it",
+                reference_published=REFERENCE_PUBLISHED)
 
 
 if __name__ == "__main__":

@@ -677,11 +677,13 @@ static brille_aux_out offset_aux(const brille_aux_out& x, uint64_t c0) {
   return o;
 }
 
-// Buckets of about kNodesPerBucket trellis nodes: k_bucket_perm orders each
-// bucket by node, so the buckets only have to be few enough that a bucketing
-// block (BIN_SPAN points) puts several points into each, and the scatter
-// writes runs of records rather than single ones.
-constexpr uint32_t kNodesPerBucket = 1024;
+// Buckets of at most kNodesPerBucket trellis nodes: k_bucket_perm orders each
+// bucket by node in LDS (kPermNodes counters), so the buckets only have to be
+// few enough that a bucketing block (BIN_SPAN points) puts runs of records
+// into each: the scatter writes a block's records of one bucket as one run,
+// and short runs cost partial-line writes.  Config 4 at 1e8 Q: 1024 nodes per
+// bucket 26.7 ms, 2048 26.6 ms, 4096 25.9 ms per call.
+constexpr uint32_t kNodesPerBucket = kPermNodes;
 static uint32_t bucket_count(const brille_grid* G) {
   const uint64_t n_nodes = std::max<uint64_t>(G->g.n_nodes, 1);
   uint32_t nb = 16;

@@ -619,6 +619,15 @@ static int rotate_point(const brille_grid_desc* g, const brille_data_desc* d,
  * first-zone fold of BrillouinZoneTrellis3::interpolate_at (src/bz_trellis.hpp:
  * 111-128: moveinto, then the trellis interpolation with no rotation).
  */
+/* wall seconds of the stages of the last brille_oracle_ir_interpolate_at on
+ * this thread: (a) fold-in, (b)+(c1) location and sums, (c2) rotation (the
+ * split of the reference's own profile, validation/profiling.md:85-93) */
+static _Thread_local double g_stage_seconds[3];
+static double now_seconds(void) { return omp_get_wtime(); }
+void brille_oracle_stage_seconds(double* out) {
+  for (int i = 0; i < 3; ++i) out[i] = g_stage_seconds[i];
+}
+
 int brille_oracle_ir_interpolate_at(const brille_grid_desc* g, const brille_data_desc* vals,
                                     const brille_data_desc* vecs, const brille_gamma_desc* gt,
                                     const brille_perm_desc* pd, const double* Q, uint64_t n,
@@ -641,6 +650,7 @@ int brille_oracle_ir_interpolate_at(const brille_grid_desc* g, const brille_data
   int result = BRILLE_OK;
   uint64_t bad = n;
 
+  double t_stage0 = now_seconds();
   /* stage (a): serial, src/bz_move.cpp:135-141, 262-285 */
   for (uint64_t i = 0; i < n; ++i) {
     if (no_move == 1) {
@@ -663,6 +673,7 @@ int brille_oracle_ir_interpolate_at(const brille_grid_desc* g, const brille_data
   memset(values_out, 0, sizeof(double) * n * B * sv);
   memset(vectors_out, 0, sizeof(double) * 2 * n * B * sc);
 
+  double t_stage1 = now_seconds();
   /* stages (b)+(c1): src/trellis_poly.hpp:330 */
   int64_t sn = (int64_t)n;
 #pragma omp parallel for schedule(dynamic) num_threads(nthreads)
@@ -718,6 +729,7 @@ int brille_oracle_ir_interpolate_at(const brille_grid_desc* g, const brille_data
     }
   }
 
+  double t_stage2 = now_seconds();
   /* stage (c2): src/bz_trellis.hpp:174-192 (interpolate_at has none) */
   int rot_status = BRILLE_OK;
 #pragma omp parallel num_threads(nthreads) if (no_move != 2)
@@ -741,6 +753,10 @@ int brille_oracle_ir_interpolate_at(const brille_grid_desc* g, const brille_data
     }
   }
 
+  double t_stage3 = now_seconds();
+  g_stage_seconds[0] = t_stage1 - t_stage0;
+  g_stage_seconds[1] = t_stage2 - t_stage1;
+  g_stage_seconds[2] = t_stage3 - t_stage2;
   /* The reference raises the first failure of the earliest stage: moveinto's
    * check over all points (src/bz_move.cpp:158), then the wedge search's first
    * point (:288-294), then a null node (rethrown, src/trellis_poly.hpp:341),

@@ -43,6 +43,8 @@ def lib():
         L.brille_oracle_approx.argtypes = [C.c_double, C.c_double, C.c_double, C.c_int]
         L.brille_oracle_find_bin.restype = C.c_uint32
         L.brille_oracle_find_bin.argtypes = [_dp, C.c_uint32, C.c_double]
+        L.brille_oracle_stage_seconds.restype = None
+        L.brille_oracle_stage_seconds.argtypes = [_dp]
         L.brille_oracle_antiphase.restype = None
         L.brille_oracle_antiphase.argtypes = [C.c_uint32, _dp, _dp, _dp]
         _lib = L
@@ -74,6 +76,14 @@ def ir_interpolate_at(tables, Q, no_move=False, nthreads=1, values=None, vectors
         1 if no_move else (2 if first_zone else 0), int(nthreads),
         vals.ctypes.data_as(_dp), vecs.ctypes.data_as(_dp), C.byref(aux), C.byref(bad))
     return rc, vals, vecs, bufs, bad.value
+
+
+def stage_seconds():
+    """Wall seconds of the last ir_interpolate_at's stages on this thread:
+    (a) fold-in, (b)+(c1) location and sums, (c2) rotation."""
+    out = np.zeros(3)
+    lib().brille_oracle_stage_seconds(out.ctypes.data_as(_dp))
+    return out
 
 
 def ir_moveinto(tables, Q):
