@@ -849,6 +849,10 @@ enum Layout : int {
   LAYOUT_WIDE = 3      // as GAMMA or SCALAR with long rows (up to kWideMax complex): one wave per (point, mode)
 };
 constexpr uint32_t kWideMax = 128;  // complex elements per row the wide phase 2 handles (two per lane)
+// modes of a point one wave sums at once in the wide phase 2: two (their loads
+// in flight together) doubled the registers and took the config-2 batch from
+// 466 to 511 ms, so one
+constexpr uint32_t kWideModes = 1;
 
 __device__ __forceinline__ float wave_sum_f(float v) {
 #pragma unroll
@@ -861,11 +865,8 @@ __device__ __forceinline__ double wave_sum_f(double v) {
   return v;
 }
 
-// Phase 2 for one (point, mode) unit by a whole wave, for rows too long for
-// one lane: lane l takes the row's complex elements l and l + 64, so every
-// row is read with coalesced loads; the Hermitian product is a wave sum.  The
-// Gamma rotation mixes the three elements of an atom, so the accumulated row
-// goes through the wave's LDS slice `lds` (kWideMax elements).
+// the wide factor table of a precision (antiphase of every slot row against
+// its block's slot-0 row), or null
 template <typename Real>
 __device__ __forceinline__ const typename Cplx<Real>::T* wide_factors(const DataConst& d);
 template <>
@@ -951,94 +952,122 @@ struct WideCache {
   uint32_t dst[2];
 };
 
-// fresh: the wave's previous unit was another point (reload the cache);
-// table: take the antiphases of points with a row-table block from the wide
-// factor table instead of the wave sums
-template <typename Real>
-__device__ inline void unit_wide(const GridConst& g, const DataConst& d, const PointRec<Real>& r, const double* q,
-                                 uint32_t b, Real* ov, typename Cplx<Real>::T* oc, typename Cplx<Real>::T* lds,
-                                 bool gamma, bool cart, WideCache<Real>& wc, bool fresh, bool table) {
+// Phase 2 for rows too long for one lane: U consecutive modes b0 .. b0 + U - 1
+// of one point by one wave (modes past B are skipped).  Lane l takes the row's
+// complex elements l and l + 64 (TWO: rows of more than 64), so every row is
+// read with coalesced loads; the slot rows and antiphase factors (from the
+// wide factor table, or Hermitian products across the wave) of all U modes
+// are requested before any is used.  Each mode is then summed, rotated
+// through the wave's LDS slice `lds` (the Gamma rotation mixes the three
+// elements of an atom) and written; `wc` keeps the point's rotation data
+// while the wave stays on it (`fresh`: the wave's previous point was another).
+template <typename Real, int U, bool TWO>
+__device__ inline void units_wide(const GridConst& g, const DataConst& d, const PointRec<Real>& r, const double* q,
+                                  uint32_t b0, Real* ov_point, typename Cplx<Real>::T* oc_point,
+                                  typename Cplx<Real>::T* lds, bool gamma, bool cart, WideCache<Real>& wc,
+                                  bool fresh, bool table) {
   using C = typename Cplx<Real>::T;
   const uint32_t l = threadIdx.x & 63u;
-  const uint32_t sc = d.sc, sv = d.sv;
-  const bool h0 = l < sc, h1 = l + 64u < sc;
+  const uint32_t sc = d.sc, sv = d.sv, B = d.B;
+  const bool h0 = l < sc, h1 = TWO && l + 64u < sc;
   const C zero = Elem<C>::zero();
-  // All eight rows are requested first and the sixteen Hermitian wave sums
-  // run side by side.  Unused tetrahedron slots repeat the pivot with weight
-  // 0 and add exact zeros.
-  C v0[8], v1[8];
-  Real vv[8];
+  C v0[U][8], v1[TWO ? U : 1][8];
+  Real vv[U][8];
+  C e[U][8];
+  const bool tab = table && r.trow != kNoCube;
+  const C* fac = wide_factors<Real>(d);
 #pragma unroll
-  for (int x = 0; x < 8; ++x) {
-    const Real* row = row_at<Real>(d, r.idx[x], b);
-    const C* rc = reinterpret_cast<const C*>(row);
-    v0[x] = h0 ? rc[l] : zero;
-    v1[x] = h1 ? rc[l + 64] : zero;
-    vv[x] = l < sv ? row[2 * sc + l] : (Real)0;
-  }
-  C e[8];
-  if (table && r.trow != kNoCube) {
-    // wave-uniform branch: one unit per wave
-    const C* fac = wide_factors<Real>(d);
+  for (int k = 0; k < U; ++k) {
+    const uint32_t b = b0 + (uint32_t)k < B ? b0 + (uint32_t)k : b0;  // modes past B repeat b0 (not stored)
 #pragma unroll
-    for (int x = 0; x < 8; ++x) e[x] = x < (int)r.nslot ? fac[r.trow + (uint32_t)x * d.B + b] : C{(Real)1, (Real)0};
-  } else {
-    const C* prow = reinterpret_cast<const C*>(row_at<Real>(d, r.pivot, b));
-    const C p0 = h0 ? prow[l] : zero, p1 = h1 ? prow[l + 64] : zero;
-    wide_antiphases<Real, 8>(p0, p1, v0, v1, e, sc);
-  }
-  C a0 = zero, a1 = zero;
-  Real va = 0;
-#pragma unroll
-  for (int x = 0; x < 8; ++x) {
-    const Real w = r.w[x];
-    Real px, py;
-    phase_elem<Real>(e[x], v0[x].x, v0[x].y, &px, &py);
-    a0.x = fma_r(w, px, a0.x);
-    a0.y = fma_r(w, py, a0.y);
-    phase_elem<Real>(e[x], v1[x].x, v1[x].y, &px, &py);
-    a1.x = fma_r(w, px, a1.x);
-    a1.y = fma_r(w, py, a1.y);
-    va = fma_r(w, vv[x], va);
-  }
-  // outputs are written once: nontemporal stores
-  typedef Real real2 __attribute__((ext_vector_type(2)));
-  if (l < sv) __builtin_nontemporal_store(va, ov + l);
-  if (!gamma) {
-    if (h0) __builtin_nontemporal_store(real2{a0.x, a0.y}, reinterpret_cast<real2*>(oc + l));
-    if (h1) __builtin_nontemporal_store(real2{a1.x, a1.y}, reinterpret_cast<real2*>(oc + l + 64));
-    return;
-  }
-  // rip_gamma_complex  src/interpolator_gamma.tpp:100-114, element by element
-  if (h0) lds[l] = a0;
-  if (h1) lds[l + 64] = a1;
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-#pragma unroll
-  for (int h = 0; h < 2; ++h) {
-    const uint32_t el = l + 64u * h;
-    if (el < sc) {
-      const uint32_t a = el / 3u, i = el - 3u * a;
-      if (fresh) {
-#pragma unroll
-        for (int j = 0; j < 3; ++j)
-          wc.R[h][j] = cart ? (Real)d.cart_rot[9 * (uint64_t)r.invrot + 3 * i + j]
-                            : (Real)g.rot[9 * (uint64_t)r.invrot + 3 * i + j];
-        wc.ph[h] = gamma_phase<Real>(d, q, r.invrot, a);
-        wc.dst[h] = 3u * d.F0[a * d.gamma_ops + r.invrot] + i;
-      }
-      const C v[3] = {lds[3 * a], lds[3 * a + 1], lds[3 * a + 2]};
-      C t = zero;
-#pragma unroll
-      for (int j = 0; j < 3; ++j) t = Elem<C>::add(t, Elem<C>::scale(v[j], wc.R[h][j]));
-      const C o = cmul(wc.ph[h], t);
-      __builtin_nontemporal_store(real2{o.x, o.y}, reinterpret_cast<real2*>(oc + wc.dst[h]));
+    for (int x = 0; x < 8; ++x) {
+      const Real* row = row_at<Real>(d, r.idx[x], b);
+      const C* rc = reinterpret_cast<const C*>(row);
+      v0[k][x] = h0 ? rc[l] : zero;
+      if constexpr (TWO) v1[k][x] = h1 ? rc[l + 64] : zero;
+      vv[k][x] = l < sv ? row[2 * sc + l] : (Real)0;
+      if (tab) e[k][x] = x < (int)r.nslot ? fac[r.trow + (uint32_t)x * B + b] : C{(Real)1, (Real)0};
     }
   }
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  if (!tab) {
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      const uint32_t b = b0 + (uint32_t)k < B ? b0 + (uint32_t)k : b0;
+      const C* prow = reinterpret_cast<const C*>(row_at<Real>(d, r.pivot, b));
+      const C p0 = h0 ? prow[l] : zero;
+      C p1 = zero;
+      if constexpr (TWO) p1 = h1 ? prow[l + 64] : zero;
+      if constexpr (TWO) {
+        wide_antiphases<Real, 8>(p0, p1, v0[k], v1[k], e[k], sc);
+      } else {
+        C z1[8];
+#pragma unroll
+        for (int x = 0; x < 8; ++x) z1[x] = zero;
+        wide_antiphases<Real, 8>(p0, zero, v0[k], z1, e[k], sc);
+      }
+    }
+  }
+  typedef Real real2 __attribute__((ext_vector_type(2)));
+#pragma unroll
+  for (int k = 0; k < U; ++k) {
+    const uint32_t b = b0 + (uint32_t)k;
+    if (b >= B) break;  // wave-uniform
+    C a0 = zero, a1 = zero;
+    Real va = 0;
+#pragma unroll
+    for (int x = 0; x < 8; ++x) {
+      const Real w = r.w[x];
+      Real px, py;
+      phase_elem<Real>(e[k][x], v0[k][x].x, v0[k][x].y, &px, &py);
+      a0.x = fma_r(w, px, a0.x);
+      a0.y = fma_r(w, py, a0.y);
+      if constexpr (TWO) {
+        phase_elem<Real>(e[k][x], v1[k][x].x, v1[k][x].y, &px, &py);
+        a1.x = fma_r(w, px, a1.x);
+        a1.y = fma_r(w, py, a1.y);
+      }
+      va = fma_r(w, vv[k][x], va);
+    }
+    Real* ov = ov_point + (uint64_t)b * sv;
+    C* oc = oc_point + (uint64_t)b * sc;
+    if (l < sv) __builtin_nontemporal_store(va, ov + l);
+    if (!gamma) {
+      if (h0) __builtin_nontemporal_store(real2{a0.x, a0.y}, reinterpret_cast<real2*>(oc + l));
+      if (h1) __builtin_nontemporal_store(real2{a1.x, a1.y}, reinterpret_cast<real2*>(oc + l + 64));
+      continue;
+    }
+    // rip_gamma_complex  src/interpolator_gamma.tpp:100-114, element by element
+    if (h0) lds[l] = a0;
+    if (h1) lds[l + 64] = a1;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+    for (int h = 0; h < (TWO ? 2 : 1); ++h) {
+      const uint32_t el = l + 64u * h;
+      if (el < sc) {
+        const uint32_t a = el / 3u, i = el - 3u * a;
+        if (fresh) {
+#pragma unroll
+          for (int j = 0; j < 3; ++j)
+            wc.R[h][j] = cart ? (Real)d.cart_rot[9 * (uint64_t)r.invrot + 3 * i + j]
+                              : (Real)g.rot[9 * (uint64_t)r.invrot + 3 * i + j];
+          wc.ph[h] = gamma_phase<Real>(d, q, r.invrot, a);
+          wc.dst[h] = 3u * d.F0[a * d.gamma_ops + r.invrot] + i;
+        }
+        const C v[3] = {lds[3 * a], lds[3 * a + 1], lds[3 * a + 2]};
+        C t = zero;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) t = Elem<C>::add(t, Elem<C>::scale(v[j], wc.R[h][j]));
+        const C o = cmul(wc.ph[h], t);
+        __builtin_nontemporal_store(real2{o.x, o.y}, reinterpret_cast<real2*>(oc + wc.dst[h]));
+      }
+    }
+    fresh = false;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
 }
 
 struct LaunchArgs {
@@ -1278,25 +1307,31 @@ __device__ __forceinline__ void tile_sums(const GridConst& g, const DataConst& d
   C* vecs = (C*)vecs_;
   const bool cart = d.vec_lu == BRILLE_LU_ANGSTROM;
   if constexpr (L == LAYOUT_WIDE) {
-    // one wave per unit; wave-uniform loop
+    // one wave per kWideModes consecutive modes of a point; wave-uniform loop
+    constexpr uint32_t U = kWideModes;
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
     const bool gamma = d.vec_rot == BRILLE_ROT_GAMMA && d.ec[1] > 0;
     C* lds = s_wide + wave * kWideMax;
     WideCache<Real> wc;
     uint32_t last = 0xFFFFFFFFu;
-    for (uint32_t u = wave; u < units; u += TB / 64) {
-      const uint32_t pl = u / B, b = u - pl * B;
+    const uint32_t CB = (B + U - 1) / U;  // mode chunks per point
+    const uint32_t chunks = npts * CB;
+    const bool two = d.sc > 64;
+    const bool tab = wide_factors<Real>(d) != nullptr;
+    for (uint32_t c = wave; c < chunks; c += TB / 64) {
+      const uint32_t pl = c / CB, b0 = (c - pl * CB) * U;
       const PointRec<Real>& r = s_rec[pl];
-      const uint64_t row = ((out_row ? (uint64_t)out_row[pl] : row0 + pl)) * B + b;
+      const uint64_t row = ((out_row ? (uint64_t)out_row[pl] : row0 + pl)) * B;
       Real* ov = vals + row * d.sv;
       C* oc = vecs + row * d.sc;
       if (r.nslot == 0) {
-        for (uint32_t e = lane; e < d.sv; e += 64) ov[e] = 0;
-        for (uint32_t e = lane; e < d.sc; e += 64) oc[e] = Elem<C>::zero();
+        const uint32_t b1 = b0 + U < B ? b0 + U : B;
+        for (uint32_t e = lane; e < (b1 - b0) * d.sv; e += 64) ov[(uint64_t)b0 * d.sv + e] = 0;
+        for (uint32_t e = lane; e < (b1 - b0) * d.sc; e += 64) oc[(uint64_t)b0 * d.sc + e] = Elem<C>::zero();
         continue;
       }
-      unit_wide<Real>(g, d, r, s_q + 3 * pl, b, ov, oc, lds, gamma, cart, wc, pl != last,
-                      wide_factors<Real>(d) != nullptr);
+      if (two) units_wide<Real, U, true>(g, d, r, s_q + 3 * pl, b0, ov, oc, lds, gamma, cart, wc, pl != last, tab);
+      else units_wide<Real, U, false>(g, d, r, s_q + 3 * pl, b0, ov, oc, lds, gamma, cart, wc, pl != last, tab);
       last = pl;
     }
     return;
