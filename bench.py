@@ -9,7 +9,9 @@ all Q.  The headline path is the binned FP32 pipeline with the learned
 tetrahedron proposal (FP64 fold-in, indices and weights; FP32/complex64 sums);
 the binned pipeline without the proposal, the fused tile kernel, the binned
 FP64 path and the plain FP64 kernel are timed beside it, and the oracle on the
-host cores.  `--config 2` times the batch of 64 uneven grids instead.
+host cores.  The line also carries configuration 2 (64 uneven grids, 1e7 Q,
+one autotuned graph-replayed batch) under "batch_config2"; `--config 2`
+makes that batch the line itself, with its per-grid baselines.
 
 Prints one JSON line.
 """
@@ -51,6 +53,10 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline (profiling runs)")
     ap.add_argument("--no-learned", action="store_true", help="config 4: skip training the tetrahedron proposal")
+    ap.add_argument("--no-batch", action="store_true", help="config 4: skip the configuration-2 batch leg")
+    ap.add_argument("--paths", default="all",
+                    help="config 4: comma-separated paths to time beside the headline (binned32, tile32, binned64; "
+                         "'none' for the headline alone, as the byte-counter runs use it)")
     ap.add_argument("--train-pairs", type=float, default=2e7, help="config 4: (q, tetrahedron) training pairs")
     ap.add_argument("--train-epochs", type=int, default=3)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -143,11 +149,18 @@ def main():
         learned["eval_proposals"] = int(tried.value)
         learned["eval_verified"] = int(hits.value)
         learned["eval_hit_rate"] = hits.value / max(1, tried.value)
-    results["binned fp32"] = timed(capi.RunOpts(0, capi.FP32, 0, capi.MEM_DEVICE, capi.KERNEL_BINNED, 0), "binned fp32")
-    results["tile fp32"] = timed(capi.RunOpts(0, capi.FP32, 0, capi.MEM_DEVICE, capi.KERNEL_TILE, 0), "tile fp32")
-    results["binned fp64"] = timed(capi.RunOpts(0, capi.FP64, 0, capi.MEM_DEVICE, capi.KERNEL_BINNED, 0), "binned fp64")
+    want = set(("binned32", "tile32", "binned64") if args.paths == "all" else
+               [x.strip() for x in args.paths.split(",") if x.strip() and x.strip() != "none"])
+    if "binned32" in want or "learned+binned fp32" not in results:
+        results["binned fp32"] = timed(capi.RunOpts(0, capi.FP32, 0, capi.MEM_DEVICE, capi.KERNEL_BINNED, 0),
+                                       "binned fp32")
+    if "tile32" in want:
+        results["tile fp32"] = timed(capi.RunOpts(0, capi.FP32, 0, capi.MEM_DEVICE, capi.KERNEL_TILE, 0), "tile fp32")
+    if "binned64" in want:
+        results["binned fp64"] = timed(capi.RunOpts(0, capi.FP64, 0, capi.MEM_DEVICE, capi.KERNEL_BINNED, 0),
+                                       "binned fp64")
     plain_res = None
-    if not args.no_plain:
+    if not args.no_plain and args.paths == "all":
         plain_res = timed(capi.RunOpts(0, capi.FP64, 0, capi.MEM_DEVICE, capi.KERNEL_PLAIN, 0), "plain fp64")
         plain_res.update(dtype="f64", bytes_per_q=24 + B * sv * 8 + B * sc * 16)
     capi.check(lib.brille_stream_destroy(stream))
@@ -220,29 +233,51 @@ def main():
         "speedup_vs_plain": (plain_res["ms_per_step"] / wall_fast) if plain_res else None,
         # the same comparison against the strongest baselines, not only the literal FP64 kernel
         "speedup_vs_binned_no_proposal": (results["binned fp32"]["ms_per_step"] / wall_fast
-                                          if head_name != "binned fp32" else 1.0),
-        "speedup_vs_tile_fp32": results["tile fp32"]["ms_per_step"] / wall_fast,
-        "speedup_vs_binned_fp64": results["binned fp64"]["ms_per_step"] / wall_fast,
+                                          if "binned fp32" in results else None),
+        "speedup_vs_tile_fp32": (results["tile fp32"]["ms_per_step"] / wall_fast if "tile fp32" in results else None),
+        "speedup_vs_binned_fp64": (results["binned fp64"]["ms_per_step"] / wall_fast
+                                   if "binned fp64" in results else None),
         "paths": results,
         "learned": learned,
         "fp32_error": fp32_error,
     }
+    if not args.no_batch:
+        # configuration 2 beside the headline, timed in the same run: the
+        # config-4 buffers are released first (the batch outputs take 152 GB)
+        del dv, dc, dQ, dev
+        res, _, _ = batch_leg(args, capi, S, with_baselines=False)
+        out["batch_config2"] = res
     print(json.dumps(out), flush=True)
 
 
-def bench_batch(args, capi, S):
-    """Configuration 2: 64 uneven grids, 1e7 Q, one persistent graph launch
-    with the autotuned schedule, against one tile-kernel call per grid."""
+def batch_leg(args, capi, S, with_baselines):
+    """Configuration 2: 64 uneven grids (vertex counts log-uniform in 2e3-2e5,
+    no payload cap: each grid's payload is generated and filled in turn,
+    keeping the FP32 rows), 1e7 Q, one persistent graph launch with the
+    autotuned schedule; with_baselines adds one tile-kernel call per grid and
+    the plain FP64 kernel grid by grid."""
     import ctypes as C
     lib = capi.lib()
-    n_q = int(args.n_q) if args.n_q is not None else 10_000_000
+    n_q = int(args.n_q) if (args.n_q is not None and args.config == 2) else 10_000_000
     t0 = time.time()
-    grids, Q, offsets = S.batch_grids(n_grids=64, n_q=n_q)
-    tables = [g.tables() for g in grids]
-    devs = [capi.Grid(t) for t in tables]
-    V = [int(t["vertices"].shape[0]) for t in tables]
-    Bm = [int(t["branches"]) for t in tables]
-    log(f"64 grids built in {time.time() - t0:.1f}s: V {min(V)}..{max(V)} (sum {sum(V)}), modes {min(Bm)}..{max(Bm)}")
+    specs = S.batch_specs(payload_cap=None)
+    Q, offsets = S.batch_queries(n_q=n_q)
+    devs, V, Bm, per_q_out, table_total = [], [], [], [], 0
+    for spec in specs:
+        g = S.batch_grid(spec)
+        vals, vecs = S.batch_payload(g, spec)
+        t = S.batch_layout_tables(g, spec)
+        dev = capi.Grid(t)
+        keep = capi.KEEP_BOTH if with_baselines else capi.KEEP_FP32
+        dev.fill_ex(t, vals.ctypes.data, vecs.ctypes.data, capi.MEM_HOST, capi.FP64, keep)
+        devs.append(dev)
+        V.append(int(vals.shape[0]))
+        Bm.append(spec["modes"])
+        per_q_out.append(spec["modes"] * 4 + spec["modes"] * 3 * spec["atoms"] * 8)
+        table_total += table_bytes(t)
+        del vals, vecs
+    log(f"64 grids built and filled in {time.time() - t0:.1f}s: V {min(V)}..{max(V)} (sum {sum(V)}), "
+        f"modes {min(Bm)}..{max(Bm)}")
     batch = capi.Batch(devs)
     dQ = capi.DeviceArray.from_host(Q)
     vals, vecs = batch.outputs(offsets, capi.FP32)
@@ -260,97 +295,74 @@ def bench_batch(args, capi, S):
     capi.check(lib.brille_device_sync())
     wall = (time.perf_counter() - w0) * 1000.0 / args.steps
     kms = [batch.time(dQ, offsets, vals, vecs, precision=capi.FP32, iters=1) for _ in range(max(1, args.steps // 2))]
-    seq = batch.time(dQ, offsets, vals, vecs, precision=capi.FP32, kernel=capi.KERNEL_TILE, iters=max(1, args.steps))
-    plain = None
-    if not args.no_plain:
-        # FP64 outputs of all 64 grids do not fit one device next to the FP32
-        # ones (config 2: 152 GB FP32, 304 GB FP64): the plain baseline runs
-        # grid by grid into one buffer sized for the largest grid
-        del vals, vecs
-        counts = np.diff(offsets.astype(np.int64))
-        vs_max = max(int(np.prod(devs[g].out_shapes(int(counts[g]))[0])) for g in range(len(devs)))
-        cs_max = max(int(np.prod(devs[g].out_shapes(int(counts[g]))[1])) for g in range(len(devs)))
-        dv64 = capi.DeviceArray((max(vs_max, 1),), np.float64)
-        dc64 = capi.DeviceArray((max(cs_max, 1),), np.complex128)
-        stream = C.c_void_p()
-        capi.check(lib.brille_stream_create(C.byref(stream)))
-        o = capi.RunOpts(0, capi.FP64, 0, capi.MEM_DEVICE, capi.KERNEL_PLAIN, 0)
-        plain = 0.0
-        for g, dev in enumerate(devs):
-            if counts[g] == 0:
-                continue
-            t = capi.KernelTimes()
-            qp = C.c_void_p(dQ.ptr.value + int(offsets[g]) * 24)
-            capi.check(lib.brille_time_ir_interpolate_at(dev.h, qp, int(counts[g]), C.byref(o), dv64.ptr, dc64.ptr, 1,
-                                                         stream, C.byref(t)))
-            plain += float(t.total_ms)
-        capi.check(lib.brille_stream_destroy(stream))
-        del dv64, dc64
-    log(f"batched fp32: {wall:.3f} ms/step wall, graph {np.mean(kms):.3f} ms; per-grid tile calls {seq:.3f} ms; "
-        f"per-grid plain fp64 {plain if plain is None else round(plain, 3)} ms")
-    per_q = np.zeros(len(devs))
-    table_total = 0
-    for g, t in enumerate(tables):
-        sv = int(sum(t["values_elements"]))
-        sc = int(sum(t["vectors_elements"]))
-        per_q[g] = 24 + Bm[g] * sv * 4 + Bm[g] * sc * 8
-        table_total += table_bytes(t)
     counts = np.diff(offsets.astype(np.int64))
-    alg = float((counts * per_q).sum()) + table_total
+    alg = float((counts * (24 + np.array(per_q_out))).sum()) + table_total
     achieved = alg / (np.mean(kms) * 1e-3) / 1e9
-    cpu = None
-    if not args.no_cpu:
-        cpu = batch_cpu_baseline(tables, Q, offsets, args.cpu_seconds)
+    res = dict(value=n_q / wall * 1e3, unit="Q-points/s", ms_per_step=wall, graph_replay_ms=float(np.mean(kms)),
+               n_q=n_q, vertices_total=int(sum(V)), vertices=[min(V), max(V)], modes=[min(Bm), max(Bm)],
+               schedule=sch, output_gb=float((counts * np.array(per_q_out)).sum() / 1e9),
+               roofline=dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
+                             frac=achieved / HBM_PEAK_GBS, kernel="k_batch<float>"))
+    log(f"batched fp32: {wall:.3f} ms/step wall, graph {np.mean(kms):.3f} ms, {achieved:.0f} GB/s")
+    if with_baselines:
+        seq = batch.time(dQ, offsets, vals, vecs, precision=capi.FP32, kernel=capi.KERNEL_TILE, iters=1)
+        res["per_grid_tile_ms"] = seq
+        res["speedup_vs_per_grid_calls"] = seq / np.mean(kms)
+        if not args.no_plain:
+            # FP64 outputs of all 64 grids do not fit one device beside the FP32
+            # ones: the plain baseline runs grid by grid into one buffer sized
+            # for the largest grid
+            del vals, vecs
+            vs_max = max(int(np.prod(devs[g].out_shapes(int(counts[g]))[0])) for g in range(len(devs)))
+            cs_max = max(int(np.prod(devs[g].out_shapes(int(counts[g]))[1])) for g in range(len(devs)))
+            dv64 = capi.DeviceArray((max(vs_max, 1),), np.float64)
+            dc64 = capi.DeviceArray((max(cs_max, 1),), np.complex128)
+            o = capi.RunOpts(0, capi.FP64, 0, capi.MEM_DEVICE, capi.KERNEL_PLAIN, 0)
+            plain = 0.0
+            for g, dev in enumerate(devs):
+                if counts[g] == 0:
+                    continue
+                t = capi.KernelTimes()
+                qp = C.c_void_p(dQ.ptr.value + int(offsets[g]) * 24)
+                capi.check(lib.brille_time_ir_interpolate_at(dev.h, qp, int(counts[g]), C.byref(o), dv64.ptr,
+                                                             dc64.ptr, 1, None, C.byref(t)))
+                plain += float(t.total_ms)
+            res["per_grid_plain_fp64_ms"] = plain
+            res["speedup_vs_plain"] = plain / wall
+            del dv64, dc64
+        log(f"per-grid tile calls {seq:.3f} ms; per-grid plain fp64 {res.get('per_grid_plain_fp64_ms')} ms")
+    return res, Q, offsets
+
+
+def bench_batch(args, capi, S):
+    """`--config 2`: the batch as the benchmark line itself."""
+    res, Q, offsets = batch_leg(args, capi, S, with_baselines=True)
     out = {
         "metric": "Q-points/s ir_interpolate_at, 1 MI355X: learned+batched vs plain, %HBM peak",
-        "value": n_q / wall * 1e3,
+        "value": res["value"],
         "unit": "Q-points/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": wall,
+        "ms_per_step": res["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"config2: 64 grids (space groups 221/139/62/14/194 cycling), V {min(V)}..{max(V)}, "
-                               f"3..72 modes, {n_q:.3g} Q split Dirichlet(0.3), one persistent graph launch",
-                   "n_q": n_q, "vertices_total": int(sum(V)), "schedule": sch, "parallelism": "single GPU"},
-        "roofline": dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
-                         traffic=None, kernel="k_batch<float>"),
-        "cpu_baseline": cpu,
-        "per_grid_tile_ms": seq,
-        "per_grid_plain_fp64_ms": plain,
-        "speedup_vs_plain": (plain / wall) if plain else None,
-        "speedup_vs_per_grid_calls": seq / np.mean(kms),
+        "config": {"workload": f"config2: 64 grids (space groups 221/139/62/14/194 cycling), V {res['vertices'][0]}.."
+                               f"{res['vertices'][1]}, 3..72 modes, {res['n_q']:.3g} Q split Dirichlet(0.3), "
+                               "one persistent graph launch",
+                   "n_q": res["n_q"], "vertices_total": res["vertices_total"], "schedule": res["schedule"],
+                   "parallelism": "single GPU"},
+        "roofline": dict(res["roofline"], traffic=None),
+        "cpu_baseline": None,
+        "per_grid_tile_ms": res.get("per_grid_tile_ms"),
+        "per_grid_plain_fp64_ms": res.get("per_grid_plain_fp64_ms"),
+        "speedup_vs_plain": res.get("speedup_vs_plain"),
+        "speedup_vs_per_grid_calls": res.get("speedup_vs_per_grid_calls"),
     }
     print(json.dumps(out), flush=True)
-
-
-def batch_cpu_baseline(tables, Q, offsets, seconds):
-    """The oracle on a bounded sample: the same fraction of every grid's points."""
-    from oracle import oracle as O
-    hw = host_cpu()
-    cores = hw["threads"]
-    counts = np.diff(offsets.astype(np.int64))
-    frac = 2e-3
-    total, tsum = 0, 0.0
-    for g, t in enumerate(tables):
-        m = int(counts[g] * frac)
-        if m == 0:
-            continue
-        a = int(offsets[g])
-        tc = time.perf_counter()
-        O.ir_interpolate_at(t, Q[a:a + m], nthreads=cores)
-        tsum += time.perf_counter() - tc
-        total += m
-        if tsum > seconds:
-            break
-    log(f"cpu baseline: {total / tsum:.4g} Q/s on {cores} threads ({total} Q)")
-    return dict(value=total / tsum, unit="Q-points/s", cores=cores, kind="port",
-                sample=f"{total} Q: the first {frac:g} of each grid's points (grids in order until {seconds:g} s), "
-                       f"FP64, {cores} OpenMP threads", cpu=hw, reference_published=REFERENCE_PUBLISHED)
 
 
 def measured_traffic(cfg, path_name):

@@ -262,6 +262,20 @@ def batch_payload(g, spec, cfg=2):
     return vals, vecs
 
 
+def batch_layout_tables(g, spec):
+    """Tables of a batch grid plus its payload layout (scalar eigenvalues, one
+    Gamma-rotating 3-vector per atom in real-lattice units) and GammaTable:
+    with batch_payload, what brille_grid_fill_ex needs."""
+    A = spec["atoms"]
+    t = dict(g.tables())
+    t.update(g.gamma_tables())
+    t.update(values_elements=[1, 0, 0], vectors_elements=[0, 3 * A, 0], values_rotates_like=0,
+             values_length_unit=3, vectors_rotates_like=2, vectors_length_unit=3, branches=3 * A)
+    for k in ("values", "vectors"):
+        t.pop(k, None)
+    return t
+
+
 def batch_queries(n_grids=64, n_q=10_000_000, box=3.0, cfg=2):
     """Q of configuration 2: n_q points uniform in [-box, box]^3 split across
     grids with Dirichlet(0.3) weights; grid g owns Q[offsets[g]:offsets[g+1]]."""

@@ -39,10 +39,7 @@ def test_full_batch_matches_the_oracle():
         g = S.batch_grid(spec)
         vals, vecs = S.batch_payload(g, spec)
         A = spec["atoms"]
-        t = dict(g.tables())
-        t.update(g.gamma_tables())
-        t.update(values_elements=[1, 0, 0], vectors_elements=[0, 3 * A, 0], values_rotates_like=0,
-                 values_length_unit=3, vectors_rotates_like=2, vectors_length_unit=3, branches=3 * A)
+        t = S.batch_layout_tables(g, spec)
         a, b = int(off[i]), int(off[i + 1])
         m = min(CHECK, b - a)
         rc, ov, oc, _, _ = O.ir_interpolate_at(t, Q[a:a + m], values=vals, vectors=vecs, nthreads=16)
