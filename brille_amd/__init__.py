@@ -32,10 +32,11 @@ from ._brille import (  # noqa: E402
     RotatesLike,
     Symmetry,
     hall_symbol,
+    spacegroup_number,
 )
 from . import capi  # noqa: E402,F401
 
 __version__ = "0.1.0"
 __all__ = ["ApproxConfig", "Basis", "BrillouinZone", "BZTrellisQdc", "Direct", "LengthUnit", "Lattice",
            "NodeType", "PointSymmetry", "Polyhedron", "Reciprocal", "RotatesLike", "Symmetry", "hall_symbol",
-           "capi"]
+           "spacegroup_number", "capi"]

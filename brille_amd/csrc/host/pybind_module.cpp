@@ -355,6 +355,7 @@ PYBIND11_MODULE(_brille, m) {
     return Lattice::from_parameters(to_v3(lengths), to_v3(angles), s, s.centring(), basis_or_empty(basis), false);
   }, "lengths"_a, "angles"_a, "symmetry_information"_a = "P 1", "basis"_a = py::none());
   m.def("hall_symbol", &hall_symbol_of, "number"_a);
+  m.def("spacegroup_number", &spacegroup_number_of, "symbol"_a);
 
   py::class_<ConvexPolyhedron>(m, "Polyhedron")
       .def_property_readonly("vertices", [](const ConvexPolyhedron& p) { return v3list(p.verts); })

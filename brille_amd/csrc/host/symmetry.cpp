@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cctype>
 #include <cfloat>
+#include <cstring>
 #include <map>
 #include <stdexcept>
 
@@ -346,33 +347,148 @@ std::string hall_symbol_of(int n) {
   return table[n];
 }
 
-Symmetry spacegroup_from_string(const std::string& s_in, Bravais* bravais) {
+// Short Hermann-Mauguin symbols of the default ITA settings (those
+// hall_symbol_of names), screw axes written with '_' (P2_1/c, P6_3/mmc) and
+// the e-glide names of the 2002 edition (Aem2, Cmce, Cmme, Ccce, Aea2).
+static const char* const kShortHM[231] = {
+    "",
+    "P1", "P-1", "P2", "P2_1", "C2", "Pm", "Pc", "Cm", "Cc", "P2/m",
+    "P2_1/m", "C2/m", "P2/c", "P2_1/c", "C2/c", "P222", "P222_1", "P2_12_12", "P2_12_12_1", "C222_1",
+    "C222", "F222", "I222", "I2_12_12_1", "Pmm2", "Pmc2_1", "Pcc2", "Pma2", "Pca2_1", "Pnc2",
+    "Pmn2_1", "Pba2", "Pna2_1", "Pnn2", "Cmm2", "Cmc2_1", "Ccc2", "Amm2", "Aem2", "Ama2",
+    "Aea2", "Fmm2", "Fdd2", "Imm2", "Iba2", "Ima2", "Pmmm", "Pnnn", "Pccm", "Pban",
+    "Pmma", "Pnna", "Pmna", "Pcca", "Pbam", "Pccn", "Pbcm", "Pnnm", "Pmmn", "Pbcn",
+    "Pbca", "Pnma", "Cmcm", "Cmce", "Cmmm", "Cccm", "Cmme", "Ccce", "Fmmm", "Fddd",
+    "Immm", "Ibam", "Ibca", "Imma", "P4", "P4_1", "P4_2", "P4_3", "I4", "I4_1",
+    "P-4", "I-4", "P4/m", "P4_2/m", "P4/n", "P4_2/n", "I4/m", "I4_1/a", "P422", "P42_12",
+    "P4_122", "P4_12_12", "P4_222", "P4_22_12", "P4_322", "P4_32_12", "I422", "I4_122", "P4mm", "P4bm",
+    "P4_2cm", "P4_2nm", "P4cc", "P4nc", "P4_2mc", "P4_2bc", "I4mm", "I4cm", "I4_1md", "I4_1cd",
+    "P-42m", "P-42c", "P-42_1m", "P-42_1c", "P-4m2", "P-4c2", "P-4b2", "P-4n2", "I-4m2", "I-4c2",
+    "I-42m", "I-42d", "P4/mmm", "P4/mcc", "P4/nbm", "P4/nnc", "P4/mbm", "P4/mnc", "P4/nmm", "P4/ncc",
+    "P4_2/mmc", "P4_2/mcm", "P4_2/nbc", "P4_2/nnm", "P4_2/mbc", "P4_2/mnm", "P4_2/nmc", "P4_2/ncm", "I4/mmm", "I4/mcm",
+    "I4_1/amd", "I4_1/acd", "P3", "P3_1", "P3_2", "R3", "P-3", "R-3", "P312", "P321",
+    "P3_112", "P3_121", "P3_212", "P3_221", "R32", "P3m1", "P31m", "P3c1", "P31c", "R3m",
+    "R3c", "P-31m", "P-31c", "P-3m1", "P-3c1", "R-3m", "R-3c", "P6", "P6_1", "P6_5",
+    "P6_2", "P6_4", "P6_3", "P-6", "P6/m", "P6_3/m", "P622", "P6_122", "P6_522", "P6_222",
+    "P6_422", "P6_322", "P6mm", "P6cc", "P6_3cm", "P6_3mc", "P-6m2", "P-6c2", "P-62m", "P-62c",
+    "P6/mmm", "P6/mcc", "P6_3/mcm", "P6_3/mmc", "P23", "F23", "I23", "P2_13", "I2_13", "Pm-3",
+    "Pn-3", "Fm-3", "Fd-3", "Im-3", "Pa-3", "Ia-3", "P432", "P4_232", "F432", "F4_132",
+    "I432", "P4_332", "P4_132", "I4_132", "P-43m", "F-43m", "I-43m", "P-43n", "F-43c", "I-43d",
+    "Pm-3m", "Pn-3n", "Pm-3n", "Pn-3m", "Fm-3m", "Fm-3c", "Fd-3m", "Fd-3c", "Im-3m", "Ia-3d"};
+
+// Short form of a Hermann-Mauguin symbol given with one token per symmetry
+// direction ("P 1 2_1/c 1", "P 2_1/n 2_1/m 2_1/a", "I 4_1/a -3 2/d",
+// "P 4/m 2_1/b m"): the rotation part of an "R/M" token is dropped except in
+// the first position of a tetragonal, trigonal or hexagonal symbol, the b
+// unique-axis monoclinic "1 X 1" becomes "X", and the tokens are joined.
+// Other settings of the monoclinic and orthorhombic groups ("P 1 1 2",
+// "P b n m") keep their form and so match no default symbol.
+static std::string short_hm(const std::string& s) {
+  std::vector<std::string> tok;
+  std::string cur;
+  for (char c : s) {
+    if (std::isspace(static_cast<unsigned char>(c))) {
+      if (!cur.empty()) tok.push_back(cur);
+      cur.clear();
+    } else {
+      cur.push_back(c);
+    }
+  }
+  if (!cur.empty()) tok.push_back(cur);
+  if (tok.size() < 2) return s;
+  std::vector<std::string> dir(tok.begin() + 1, tok.end());
+  if (dir.size() == 3 && dir[0] == "1" && dir[2] == "1" && dir[1] != "1") dir = {dir[1]};
+  const bool cubic = dir.size() >= 2 && (dir[1] == "3" || dir[1] == "-3");
+  const bool ortho = dir.size() == 3 && std::all_of(dir.begin(), dir.end(), [](const std::string& t) {
+                       return t.find('/') != std::string::npos && (t[0] == '2');
+                     });
+  std::string out = tok[0];
+  for (size_t i = 0; i < dir.size(); ++i) {
+    std::string t = dir[i];
+    const size_t sl = t.find('/');
+    if (sl != std::string::npos && (i > 0 || cubic || ortho)) t = t.substr(sl + 1);
+    out += t;
+  }
+  return out;
+}
+
+static std::string without(const std::string& s, const char* drop) {
+  std::string out;
+  for (char c : s)
+    if (!std::strchr(drop, c)) out.push_back(c);
+  return out;
+}
+
+// ITA number of a Hermann-Mauguin symbol of a default setting, 0 if none.
+// Matched with spaces ignored, first as written (with '_' for screw axes),
+// then, when `loose`, with '_' ignored too (P21/c, P63/mmc).
+static int ita_of_hm(const std::string& s, bool loose) {
+  const std::string k1 = without(s, " \t"), k2 = without(short_hm(s), " \t");
+  for (int n = 1; n <= 230; ++n)
+    if (k1 == kShortHM[n] || k2 == kShortHM[n]) return n;
+  if (loose) {
+    const std::string l1 = without(k1, "_"), l2 = without(k2, "_");
+    for (int n = 1; n <= 230; ++n) {
+      const std::string t = without(kShortHM[n], "_");
+      if (l1 == t || l2 == t) return n;
+    }
+  }
+  return 0;
+}
+
+int spacegroup_number_of(const std::string& s_in) {
   std::string s = s_in;
-  // trim
   while (!s.empty() && std::isspace(static_cast<unsigned char>(s.front()))) s.erase(s.begin());
   while (!s.empty() && std::isspace(static_cast<unsigned char>(s.back()))) s.pop_back();
-  bool all_digits = !s.empty() && std::all_of(s.begin(), s.end(), [](char c) { return std::isdigit(static_cast<unsigned char>(c)); });
+  if (!s.empty() && std::all_of(s.begin(), s.end(), [](char c) { return std::isdigit(static_cast<unsigned char>(c)); })) {
+    const int n = std::stoi(s);
+    return n >= 1 && n <= 230 ? n : 0;
+  }
+  for (int n = 1; n <= 230; ++n)
+    if (s == hall_symbol_of(n)) return n;
+  if (s == "P 2") return 3;
+  if (int n = ita_of_hm(s, false)) return n;
+  try {
+    (void)hall_generators(s);
+    return 0;  // a Hall symbol of a setting the default table does not hold
+  } catch (const std::exception&) {
+  }
+  return ita_of_hm(s, true);
+}
+
+// The order follows the reference's lookup (src/spg_database.cpp:622-650,
+// src/lattice_dual.hpp:510-526): an ITA number; a tabulated Hall symbol; a
+// Hermann-Mauguin symbol (full, spaced or short); otherwise the string is
+// parsed as a Hall symbol.  The last resort, not in the reference, accepts a
+// short symbol written without '_'.
+Symmetry spacegroup_from_string(const std::string& s_in, Bravais* bravais) {
+  std::string s = s_in;
+  while (!s.empty() && std::isspace(static_cast<unsigned char>(s.front()))) s.erase(s.begin());
+  while (!s.empty() && std::isspace(static_cast<unsigned char>(s.back()))) s.pop_back();
+  const bool all_digits = !s.empty() && std::all_of(s.begin(), s.end(), [](char c) { return std::isdigit(static_cast<unsigned char>(c)); });
   std::string hall;
+  bool tabulated = false;
+  for (int n = 1; n <= 230 && !tabulated; ++n) tabulated = s == hall_symbol_of(n);
+  // "P 2" is also the Hall symbol of the c unique-axis setting of P2, which
+  // the reference's Hall-symbol match finds before any Hermann-Mauguin name
+  tabulated = tabulated || s == "P 2";
   if (all_digits) {
     hall = hall_symbol_of(std::stoi(s));
     if (hall.empty()) throw std::runtime_error("unknown space group number " + s);
+  } else if (tabulated) {
+    hall = s;
+  } else if (int n = ita_of_hm(s, false)) {
+    hall = hall_symbol_of(n);
   } else {
-    std::string key;
-    for (char c : s)
-      if (!std::isspace(static_cast<unsigned char>(c)) && c != '_') key.push_back(c);
-    static const std::map<std::string, int> hm{
-        {"P1", 1}, {"P-1", 2}, {"P2/m", 10}, {"C2/m", 12}, {"P21/c", 14}, {"C2/c", 15}, {"P222", 16},
-        {"Pmmm", 47}, {"Pnma", 62}, {"Cmcm", 63}, {"Cmce", 64}, {"Cmca", 64}, {"Fmmm", 69}, {"Fddd", 70},
-        {"Immm", 71}, {"P4/mmm", 123}, {"P4/nmm", 129}, {"I4/mmm", 139}, {"I41/amd", 141}, {"P-3m1", 164},
-        {"R-3m", 166}, {"R-3c", 167}, {"P6/mmm", 191}, {"P63/mmc", 194}, {"P3221", 154}, {"P3121", 152},
-        {"Pm-3m", 221}, {"Fm-3m", 225}, {"Fd-3m", 227}, {"Im-3m", 229}, {"Ia-3d", 230}, {"F-43m", 216},
-        {"P6_3/mmc", 194}, {"P2_1/c", 14}};
-    auto it = hm.find(key);
-    if (it != hm.end()) hall = hall_symbol_of(it->second);
-    else hall = s;  // treat as a Hall symbol
+    try {
+      return hall_generators(s, bravais).generate();
+    } catch (const std::exception& e) {
+      if (int m = ita_of_hm(s, true)) hall = hall_symbol_of(m);
+      else throw std::runtime_error("'" + s + "' is neither a Hall symbol (" + e.what() +
+                                    ") nor the Hermann-Mauguin symbol of a default setting");
+    }
   }
-  Symmetry g = hall_generators(hall, bravais).generate();
-  return g;
+  return hall_generators(hall, bravais).generate();
 }
 
 // ---------------------------------------------------------------------------

@@ -50,6 +50,10 @@ Symmetry hall_generators(const std::string& hall, Bravais* bravais = nullptr);
 Symmetry spacegroup_from_string(const std::string& s, Bravais* bravais = nullptr);
 // Default-setting Hall symbol of an ITA number, "" when not tabulated.
 std::string hall_symbol_of(int ita_number);
+// ITA number named by a space-group string (number, Hall symbol of a default
+// setting, Hermann-Mauguin symbol); 0 when the string names another setting
+// or nothing.
+int spacegroup_number_of(const std::string& symbol);
 
 int rotation_order(const IM3& R);
 int isometry_value(const IM3& R);  // +-order, negative for improper
