@@ -27,7 +27,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-I", os.path.join(ROOT, "include"),
 ]
-HIP_SOURCES = ["capi.hip", "run.hip", "binned.hip", "batch.hip", "proposal.hip", "noalloc.hip"]
+HIP_SOURCES = ["capi.hip", "run.hip", "binned.hip", "batch.hip", "proposal.hip", "noalloc.hip", "sort.hip"]
 HOST_SOURCES = ["symmetry.cpp", "lattice.cpp", "polyhedron.cpp", "bz.cpp", "trellis.cpp", "grid.cpp",
                 "pybind_module.cpp"]
 

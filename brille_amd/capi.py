@@ -72,6 +72,14 @@ class FillOpts(C.Structure):
     _fields_ = [("memory", C.c_int32), ("data_type", C.c_int32), ("keep", C.c_int32)]
 
 
+class CostInfo(C.Structure):
+    _fields_ = [("scalar_cost", C.c_int32), ("vector_cost", C.c_int32), ("weights", C.c_double * 3)]
+
+
+def cost_info(scalar_cost=0, vector_cost=0, weights=(1.0, 1.0, 1.0)):
+    return CostInfo(int(scalar_cost), int(vector_cost), (C.c_double * 3)(*[float(w) for w in weights]))
+
+
 class RunOpts(C.Structure):
     _fields_ = [("no_move", C.c_int32), ("precision", C.c_int32), ("use_proposal", C.c_int32),
                 ("memory", C.c_int32), ("kernel", C.c_int32), ("check", C.c_int32)]
@@ -109,6 +117,8 @@ SIGNATURES = {
     "brille_grid_destroy": (C.c_int, [C.c_void_p]),
     "brille_grid_fill_ex": (C.c_int, [C.c_void_p, C.POINTER(DataDesc), C.POINTER(DataDesc), C.POINTER(GammaDesc),
                                       C.POINTER(PermDesc), C.POINTER(FillOpts)]),
+    "brille_grid_sort": (C.c_int, [C.c_void_p, C.POINTER(CostInfo), C.POINTER(CostInfo)]),
+    "brille_grid_permutations": (C.c_int, [C.c_void_p, _u64p, _up, _u64p, _up, _up]),
     "brille_bytes_per_point": (C.c_uint64, [C.c_void_p, C.c_int32]),
     "brille_ir_interpolate_at": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(RunOpts), C.c_void_p,
                                            C.c_void_p, C.POINTER(AuxOut), C.c_void_p]),
@@ -123,7 +133,8 @@ SIGNATURES = {
                                    C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_void_p]),
     "brille_batch_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(RunOpts),
                                         C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_void_p]),
-    "brille_batch_schedule": (C.c_int, [C.c_void_p, _up, _up, _up, _up]),
+    "brille_batch_schedule": (C.c_int, [C.c_void_p, _up, _up, _up, _up, _u64p]),
+    "brille_batch_set_schedule": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64]),
     "brille_batch_destroy": (C.c_int, [C.c_void_p]),
     "brille_proposal_train": (C.c_int, [C.c_void_p, C.POINTER(TrainOpts), C.POINTER(TrainReport)]),
     "brille_proposal_stats": (C.c_int, [C.c_void_p, _u64p, _u64p]),
@@ -391,6 +402,23 @@ class Grid:
                                         C.byref(d.gdesc) if d.gdesc is not None else None,
                                         C.byref(d.pdesc) if d.pdesc is not None else None, C.byref(o)))
 
+    def sort(self, values_cost=None, vectors_cost=None):
+        """brille_grid_sort: mode permutations of every connected vertex pair."""
+        check(lib().brille_grid_sort(self.h, C.byref(values_cost) if values_cost is not None else None,
+                                     C.byref(vectors_cost) if vectors_cost is not None else None))
+
+    def permutations(self):
+        """(keys, perm_index, perms) of the grid's permutation table."""
+        nk, npm = C.c_uint64(), C.c_uint32()
+        L = lib()
+        check(L.brille_grid_permutations(self.h, C.byref(nk), C.byref(npm), None, None, None))
+        keys = np.zeros(nk.value, np.uint64)
+        index = np.zeros(nk.value, np.uint32)
+        perms = np.zeros((npm.value, self.branches), np.uint32)
+        check(L.brille_grid_permutations(self.h, C.byref(nk), C.byref(npm), keys.ctypes.data_as(_u64p),
+                                         index.ctypes.data_as(_up), perms.ctypes.data_as(_up)))
+        return keys, index, perms
+
     @property
     def branches(self):
         return self.data.vdesc.branches
@@ -523,10 +551,16 @@ class Batch:
                                           None, C.byref(t)))
         return t.total_ms
 
+    def set_schedule(self, tile_points=256, blocks_per_cu=4, tiles_per_grab=1, order=1, binned_min=None):
+        check(lib().brille_batch_set_schedule(self.h, tile_points, blocks_per_cu, tiles_per_grab, order,
+                                              2 ** 64 - 1 if binned_min is None else int(binned_min)))
+
     def schedule(self):
         v = [C.c_uint32() for _ in range(4)]
-        check(lib().brille_batch_schedule(self.h, *[C.byref(x) for x in v]))
-        return dict(tile_points=v[0].value, blocks=v[1].value, tiles_per_grab=v[2].value, order=v[3].value)
+        bmin = C.c_uint64()
+        check(lib().brille_batch_schedule(self.h, *[C.byref(x) for x in v], C.byref(bmin)))
+        return dict(tile_points=v[0].value, blocks=v[1].value, tiles_per_grab=v[2].value, order=v[3].value,
+                    binned_min=None if bmin.value == 2 ** 64 - 1 else int(bmin.value))
 
     def __del__(self):
         try:

@@ -175,6 +175,9 @@ struct Schedule {
   uint32_t blocks_per_cu = 4;
   uint32_t grab = 1;
   uint32_t order = 1;        // 0: as given, 1: costliest points first
+  // grids given at least this many points run their own binned pipeline
+  // after the persistent kernel (their rows are then read in node order)
+  uint64_t binned_min = ~0ull;
 };
 
 }  // namespace brille_amd
@@ -257,7 +260,7 @@ static int stage_slots(brille_batch* b, const double* Q, const uint64_t* offsets
   uint64_t tiles = 0;
   for (uint32_t g : order) {
     const uint64_t n = offsets[g + 1] - offsets[g];
-    if (n == 0) continue;
+    if (n == 0 || n >= sc.binned_min) continue;
     BatchSlot sl{};
     sl.Q = Q + 3 * offsets[g];
     sl.vals = vals[g];
@@ -359,6 +362,9 @@ static int ensure_batch_tables(brille_batch* b, int precision, hipStream_t s) {
   return BRILLE_OK;
 }
 
+static int launch_binned_grids(brille_batch* b, const double* Q, const uint64_t* offsets, const brille_run_opts& o,
+                               void* const* vals, void* const* vecs, hipStream_t s, const Schedule& sc);
+
 static int run_with(brille_batch* b, const double* Q, const uint64_t* offsets, const brille_run_opts& o,
                     void* const* vals, void* const* vecs, hipStream_t s, const Schedule& sc) {
   int rc = ensure_batch_tables(b, o.precision, s);
@@ -369,6 +375,24 @@ static int run_with(brille_batch* b, const double* Q, const uint64_t* offsets, c
   rc = graph_for(b, o.precision, sc, &exec);
   if (rc != BRILLE_OK) return rc;
   BATCH_TRY(hipGraphLaunch(exec, s));
+  return launch_binned_grids(b, Q, offsets, o, vals, vecs, s, sc);
+}
+
+// The large grids after the graph (whose reset clears every grid's error
+// record), each a binned call on the same stream; errors are read by the
+// caller as for the others.
+static int launch_binned_grids(brille_batch* b, const double* Q, const uint64_t* offsets, const brille_run_opts& o,
+                               void* const* vals, void* const* vecs, hipStream_t s, const Schedule& sc) {
+  int rc = BRILLE_OK;
+  brille_run_opts ob = o;
+  ob.kernel = BRILLE_KERNEL_BINNED;
+  ob.check = 0;
+  for (size_t g = 0; g < b->grids.size(); ++g) {
+    const uint64_t n = offsets[g + 1] - offsets[g];
+    if (n == 0 || n < sc.binned_min) continue;
+    rc = run_interpolate(b->grids[g], Q + 3 * offsets[g], n, &ob, vals[g], vecs[g], nullptr, s, nullptr, 0);
+    if (rc != BRILLE_OK) return fail(rc, "grid " + std::to_string(g) + ": " + std::string(g_last_error_text()));
+  }
   return BRILLE_OK;
 }
 
@@ -472,7 +496,7 @@ int brille_batch_autotune(brille_batch* b, const double* Q, const uint64_t* offs
       for (uint32_t grab : {1u, 4u})
         for (uint32_t order : {0u, 1u}) {
           if (rc != BRILLE_OK) break;
-          Schedule sc{tile, bpc, grab, order};
+          Schedule sc{tile, bpc, grab, order, ~0ull};
           float ms = 0.f;
           rc = time_one(sc, &ms);
           if (rc == BRILLE_OK && (best_ms < 0.f || ms < best_ms)) {
@@ -480,6 +504,19 @@ int brille_batch_autotune(brille_batch* b, const double* Q, const uint64_t* offs
             best = sc;
           }
         }
+  // then the share of the work the binned pipeline takes, with that schedule
+  const Schedule persistent = best;
+  for (uint64_t bmin : {1ull << 20, 1ull << 18, 1ull << 16, 1ull << 14}) {
+    if (rc != BRILLE_OK) break;
+    Schedule sc = persistent;
+    sc.binned_min = bmin;
+    float ms = 0.f;
+    rc = time_one(sc, &ms);
+    if (rc == BRILLE_OK && ms < best_ms) {
+      best_ms = ms;
+      best = sc;
+    }
+  }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (rc != BRILLE_OK) return rc;
@@ -490,12 +527,24 @@ int brille_batch_autotune(brille_batch* b, const double* Q, const uint64_t* offs
 }
 
 int brille_batch_schedule(const brille_batch* b, uint32_t* tile_points, uint32_t* blocks, uint32_t* tiles_per_grab,
-                          uint32_t* order) {
+                          uint32_t* order, uint64_t* binned_min) {
   if (!b) return fail(BRILLE_ERR_INVALID_ARGUMENT, "null batch");
   if (tile_points) *tile_points = b->sched.tile;
   if (blocks) *blocks = std::max(1u, b->n_cu * b->sched.blocks_per_cu);
   if (tiles_per_grab) *tiles_per_grab = b->sched.grab;
   if (order) *order = b->sched.order;
+  if (binned_min) *binned_min = b->sched.binned_min;
+  return BRILLE_OK;
+}
+
+int brille_batch_set_schedule(brille_batch* b, uint32_t tile_points, uint32_t blocks_per_cu, uint32_t tiles_per_grab,
+                              uint32_t order, uint64_t binned_min) {
+  if (!b) return fail(BRILLE_ERR_INVALID_ARGUMENT, "null batch");
+  if ((tile_points != 128 && tile_points != 256) || blocks_per_cu == 0 || blocks_per_cu > 64 || tiles_per_grab == 0 ||
+      tiles_per_grab > 64 || order > 1)
+    return fail(BRILLE_ERR_INVALID_ARGUMENT, "schedule out of range (tile 128 or 256, 1-64 blocks per CU, 1-64 tiles per grab, order 0 or 1)");
+  GridLocks member_locks(b->lock_set());
+  b->sched = Schedule{tile_points, blocks_per_cu, tiles_per_grab, order, binned_min};
   return BRILLE_OK;
 }
 
@@ -546,6 +595,8 @@ int brille_time_batch_run(brille_batch* b, const double* Q, const uint64_t* offs
       if (rc != BRILLE_OK) return rc;
     } else {
       BATCH_TRY(hipGraphLaunch(exec, s));
+      rc = launch_binned_grids(b, Q, offsets, o, values_out, vectors_out, s, b->sched);
+      if (rc != BRILLE_OK) return rc;
     }
   }
   (void)hipEventRecord(e1, s);

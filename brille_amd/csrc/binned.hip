@@ -42,6 +42,11 @@ struct FoldRec {
 // least kPropMinCand tetrahedra (in smaller nodes the exact scan costs no
 // more than verifying a proposal, and the network is not run)
 constexpr uint32_t kMetaPoly = 1u << 24;
+// the same flag on the node words the node-order pass reads (node indices
+// stay below 2^31, which the launch checks) and on the perm entries it
+// writes (positions within a pass stay below 2^31)
+constexpr uint32_t kNodePoly = 1u << 31;
+constexpr uint32_t kPermPoly = 1u << 31;
 constexpr uint32_t kPropMinCand = 8;
 __device__ __forceinline__ uint32_t meta_status(uint32_t meta) { return (meta >> 16) & 255u; }
 struct SortRec {
@@ -56,36 +61,29 @@ __device__ __forceinline__ uint32_t bucket_of(uint32_t node, uint32_t meta, uint
   return meta_status(meta) != BRILLE_PT_OK ? nb : (uint32_t)(((uint64_t)node * nb) / n_nodes);
 }
 
-// poly_list/poly_count (nullable): the points for the learned proposal, those
-// that folded into polyhedron nodes of at least kPropMinCand tetrahedra
-// (appended per wave: ballot, prefix count, one atomic)
+// mark_poly: flag (kMetaPoly) the points for the learned proposal, those
+// that folded into polyhedron nodes of at least kPropMinCand tetrahedra; the
+// node-order pass lists them
 // Six waves per SIMD (a few spilled registers) hide more of the FP64 latency
 // than the four that 124 VGPRs allow: 9 % less time.
 template <bool Fast>
 __global__ __launch_bounds__(BIN_BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void k_bin_fold(GridConst g, const double* Q, uint32_t n, int fold,
                                                         FoldRec* __restrict__ rec, uint32_t* __restrict__ hist,
                                                         uint32_t nblk, uint32_t nb,
-                                                        ErrRec* __restrict__ err, uint64_t base,
-                                                        uint32_t* __restrict__ poly_list,
-                                                        uint32_t* __restrict__ poly_count) {
+                                                        ErrRec* __restrict__ err, uint64_t base, int mark_poly) {
   extern __shared__ uint32_t h[];  // nb + 1 counters
-  __shared__ uint32_t s_poly[BIN_SPAN];
-  __shared__ uint32_t s_npoly, s_at;
   for (uint32_t k = threadIdx.x; k <= nb; k += BIN_BLOCK) h[k] = 0;
-  if (threadIdx.x == 0) s_npoly = 0;
   __syncthreads();
   const uint32_t b0 = blockIdx.x * BIN_SPAN;
-  const uint32_t lane = threadIdx.x & 63u;
   for (int i = 0; i < BIN_PPT; ++i) {
     const uint32_t p = b0 + i * BIN_BLOCK + threadIdx.x;
-    bool poly = false;
     if (p < n) {
       double Qp[3] = {Q[3 * (uint64_t)p], Q[3 * (uint64_t)p + 1], Q[3 * (uint64_t)p + 2]};
       Located L;
       fold_node<Fast>(g, Qp, fold, L);
       if (L.status != BRILLE_PT_OK) report_point(err, base + p, L.status);
-      poly = L.status == BRILLE_PT_OK && g.node_type[L.node] == BRILLE_NODE_POLY;
-      if (poly && poly_list) {
+      bool poly = false;
+      if (mark_poly && L.status == BRILLE_PT_OK && g.node_type[L.node] == BRILLE_NODE_POLY) {
         const uint32_t pi = g.node_index[L.node];
         poly = g.poly_off[pi + 1] - g.poly_off[pi] >= kPropMinCand;
       }
@@ -96,25 +94,9 @@ __global__ __launch_bounds__(BIN_BLOCK) __attribute__((amdgpu_waves_per_eu(6))) 
       rec[p] = r;
       atomicAdd(&h[bucket_of(r.node, r.meta, nb, g.n_nodes)], 1u);
     }
-    if (poly_list) {
-      // block-local list first: one global atomic per block, not per wave
-      const unsigned long long mask = __ballot(poly);
-      if (mask) {
-        const uint32_t leader = (uint32_t)__ffsll((long long)mask) - 1u;
-        uint32_t at = 0;
-        if (lane == leader) at = atomicAdd(&s_npoly, (uint32_t)__popcll(mask));
-        at = __shfl(at, (int)leader);
-        if (poly) s_poly[at + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull))] = p;
-      }
-    }
   }
   __syncthreads();
   for (uint32_t k = threadIdx.x; k <= nb; k += BIN_BLOCK) hist[(uint64_t)k * nblk + blockIdx.x] = h[k];
-  if (poly_list) {
-    if (threadIdx.x == 0) s_at = s_npoly ? atomicAdd(poly_count, s_npoly) : 0u;
-    __syncthreads();
-    for (uint32_t k = threadIdx.x; k < s_npoly; k += BIN_BLOCK) poly_list[s_at + k] = s_poly[k];
-  }
 }
 
 // exclusive scan of 256 values held one per thread; returns the thread's
@@ -192,7 +174,7 @@ __global__ __launch_bounds__(BIN_BLOCK) void k_scan_apply(uint32_t* a, uint64_t 
 
 __global__ __launch_bounds__(BIN_BLOCK) void k_bin_scatter(const FoldRec* rec, uint32_t n, const uint32_t* off,
                                                            uint32_t nblk, uint32_t nb, uint64_t n_nodes, SortRec* out,
-                                                           uint32_t* out_node, const uint32_t* hint) {
+                                                           uint32_t* out_node) {
   extern __shared__ uint32_t s[];  // base[nb + 1], count[nb + 1]
   uint32_t* base = s;
   uint32_t* cnt = s + nb + 1;
@@ -222,9 +204,9 @@ __global__ __launch_bounds__(BIN_BLOCK) void k_bin_scatter(const FoldRec* rec, u
       o.node = r.node;
       o.meta = r.meta;
       o.index = p;
-      o.hint = (hint && (r.meta & kMetaPoly)) ? hint[p] : kNoTet;
+      o.hint = kNoTet;
       out[base[k] + j] = o;
-      out_node[base[k] + j] = r.node;
+      out_node[base[k] + j] = r.node | ((r.meta & kMetaPoly) ? kNodePoly : 0u);
     }
     }
   }
@@ -235,7 +217,8 @@ __global__ __launch_bounds__(BIN_BLOCK) void k_bin_scatter(const FoldRec* rec, u
 // in node order, so that the points of one cube are summed together and its
 // rows are read from L1.  Within a node the order is whatever the LDS atomics
 // give (the points are independent).  Buckets spanning more than kPermNodes
-// nodes, and the failed-point bucket, keep their order.
+// nodes, and the failed-point bucket, keep their order.  Each entry carries
+// the record's proposal flag (kPermPoly, from kNodePoly).
 constexpr uint32_t kPermNodes = 4096;
 constexpr int PERM_BLOCK = 1024;
 __global__ __launch_bounds__(PERM_BLOCK) void k_bucket_perm(const uint32_t* node, uint32_t m, const uint32_t* off,
@@ -249,12 +232,12 @@ __global__ __launch_bounds__(PERM_BLOCK) void k_bucket_perm(const uint32_t* node
   const uint64_t hi = b < nb ? ((uint64_t)(b + 1) * n_nodes + nb - 1) / nb : lo;
   const uint32_t nn = (uint32_t)(hi - lo);
   if (b == nb || nn == 0 || nn > kPermNodes) {
-    for (uint32_t i = s0 + threadIdx.x; i < s1; i += PERM_BLOCK) perm[i] = i;
+    for (uint32_t i = s0 + threadIdx.x; i < s1; i += PERM_BLOCK) perm[i] = i | (node[i] & kNodePoly);
     return;
   }
   for (uint32_t k = threadIdx.x; k < nn; k += PERM_BLOCK) cnt[k] = 0;
   __syncthreads();
-  for (uint32_t i = s0 + threadIdx.x; i < s1; i += PERM_BLOCK) atomicAdd(&cnt[node[i] - (uint32_t)lo], 1u);
+  for (uint32_t i = s0 + threadIdx.x; i < s1; i += PERM_BLOCK) atomicAdd(&cnt[(node[i] & ~kNodePoly) - (uint32_t)lo], 1u);
   __syncthreads();
   // exclusive scan of cnt[0, nn): each thread a contiguous run, then the runs
   const uint32_t per = (nn + PERM_BLOCK - 1) / PERM_BLOCK;
@@ -269,16 +252,64 @@ __global__ __launch_bounds__(PERM_BLOCK) void k_bucket_perm(const uint32_t* node
     acc += c;
   }
   __syncthreads();
-  for (uint32_t i = s0 + threadIdx.x; i < s1; i += PERM_BLOCK) perm[atomicAdd(&cnt[node[i] - (uint32_t)lo], 1u)] = i;
+  for (uint32_t i = s0 + threadIdx.x; i < s1; i += PERM_BLOCK) {
+    const uint32_t nd = node[i];
+    perm[atomicAdd(&cnt[(nd & ~kNodePoly) - (uint32_t)lo], 1u)] = i | (nd & kNodePoly);
+  }
 }
 
-// The learned proposal for the listed points only (those in polyhedron
-// nodes): each wave takes 16 list entries at a time, stages their features
-// and candidate ranges in its LDS slice, runs the network and writes the
-// proposed tetrahedron to hint[point].  Grid-stride over the list, whose
-// length is read on the device.
-__global__ __launch_bounds__(256) void k_bin_propose(GridConst g, ProposalConst pc, const FoldRec* rec,
-                                                     const uint32_t* list, const uint32_t* count, uint32_t* hint) {
+// The flagged records (kPermPoly) of perm, listed in perm order: each block
+// compacts its 4096 consecutive entries in LDS and appends them with one
+// atomic, so the list runs in node order within each block's share.
+constexpr int LIST_PPT = 16;
+constexpr uint32_t LIST_SPAN = BIN_BLOCK * LIST_PPT;
+__global__ __launch_bounds__(BIN_BLOCK) void k_poly_list(const uint32_t* perm, uint32_t m, uint32_t* __restrict__ list,
+                                                        uint32_t* __restrict__ count) {
+  __shared__ uint32_t s_list[LIST_SPAN];
+  __shared__ uint32_t s_n, s_at;
+  if (threadIdx.x == 0) s_n = 0;
+  __syncthreads();
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  const uint64_t b0 = (uint64_t)blockIdx.x * LIST_SPAN;
+  // each wave a contiguous quarter of the span, 64 entries at a time, so the
+  // LDS list keeps perm order within each wave's quarter
+  __shared__ uint32_t s_wn[BIN_BLOCK / 64];
+  uint32_t local = 0;
+  // pass 1: count per wave
+  const uint64_t w0 = b0 + (uint64_t)wave * (LIST_SPAN / (BIN_BLOCK / 64));
+  for (uint32_t k = 0; k < LIST_SPAN / BIN_BLOCK; ++k) {
+    const uint64_t j = w0 + (uint64_t)k * 64u + lane;
+    const bool poly = j < m && (perm[j] & kPermPoly) != 0u;
+    local += (uint32_t)__popcll(__ballot(poly));
+  }
+  if (lane == 0) s_wn[wave] = local;
+  __syncthreads();
+  uint32_t at = 0;
+  for (uint32_t w = 0; w < wave; ++w) at += s_wn[w];
+  // pass 2: place, in order within the wave's quarter
+  for (uint32_t k = 0; k < LIST_SPAN / BIN_BLOCK; ++k) {
+    const uint64_t j = w0 + (uint64_t)k * 64u + lane;
+    const uint32_t e = j < m ? perm[j] : 0u;
+    const bool poly = (e & kPermPoly) != 0u;
+    const unsigned long long mask = __ballot(poly);
+    if (poly) s_list[at + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull))] = e & ~kPermPoly;
+    at += (uint32_t)__popcll(mask);
+  }
+  if (threadIdx.x == BIN_BLOCK - 1) s_n = at;  // the last wave ends at the block's total
+  __syncthreads();
+  if (threadIdx.x == 0) s_at = s_n ? atomicAdd(count, s_n) : 0u;
+  __syncthreads();
+  for (uint32_t k = threadIdx.x; k < s_n; k += BIN_BLOCK) list[s_at + k] = s_list[k];
+}
+
+// The learned proposal for the listed records: each wave takes 16 list
+// entries at a time (mostly of one node, so the candidate rows the chunk
+// reads are mostly the same rows), stages their features and candidate
+// ranges in its LDS slice, runs the network and writes the proposed
+// tetrahedron to the record's hint.  Grid-stride over the list, whose length
+// is read on the device.
+__global__ __launch_bounds__(256) void k_bin_propose(GridConst g, ProposalConst pc, SortRec* srt,
+                                                     const uint32_t* list, const uint32_t* count) {
   // chunks of 16 listed points per wave, four waves per block, grid-stride
   // over the list; blocks past the list's end leave before copying weights
   const uint32_t n = *count;
@@ -301,16 +332,20 @@ __global__ __launch_bounds__(256) void k_bin_propose(GridConst g, ProposalConst 
   // wave-uniform loop; no block-wide barrier follows
   for (uint32_t c0 = (blockIdx.x * 4u + wave) * 16u; c0 < n; c0 += gridDim.x * 64u) {
   const uint32_t k = n - c0 < 16u ? n - c0 : 16u;
+  uint32_t rec = 0;
   if (lane < k) {
-    const FoldRec r = rec[list[c0 + lane]];
+    rec = list[c0 + lane];
+    const SortRec& r = srt[rec];
+    const double q[3] = {r.q[0], r.q[1], r.q[2]};
+    const uint32_t nd = r.node;
     double x[3];
-    mat_vec(x, g.outer_to_xyz, r.q);
+    mat_vec(x, g.outer_to_xyz, q);
     float f[3];
-    prop_features(g, x, r.node, f);
+    prop_features(g, x, nd, f);
     s_feat[wave][3 * lane] = f[0];
     s_feat[wave][3 * lane + 1] = f[1];
     s_feat[wave][3 * lane + 2] = f[2];
-    const uint32_t pi = g.node_index[r.node];
+    const uint32_t pi = g.node_index[nd];
     const uint32_t a = g.poly_off[pi];
     s_t0[wave][lane] = a;
     s_nc[wave][lane] = g.poly_off[pi + 1] - a;
@@ -322,7 +357,7 @@ __global__ __launch_bounds__(256) void k_bin_propose(GridConst g, ProposalConst 
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  if (lane < k) hint[list[c0 + lane]] = s_out[wave][lane];
+  if (lane < k) srt[rec].hint = s_out[wave][lane];
   // the next chunk overwrites this wave's LDS slices
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -378,7 +413,7 @@ __global__ __launch_bounds__(TILE) void k_bin_sums(GridConst g, DataConst d, Pro
   P.status = BRILLE_PT_OK;
   P.cube = kNoCube;
   if (live) {
-    sr = srt[perm[p]];
+    sr = srt[perm[p] & ~kPermPoly];
     P.q[0] = sr.q[0]; P.q[1] = sr.q[1]; P.q[2] = sr.q[2];
     P.tau[0] = P.tau[1] = P.tau[2] = 0;
     P.node = sr.node;
@@ -715,6 +750,17 @@ void zero_words(void* p, uint32_t n, hipStream_t s) {
   if (n) hipLaunchKernelGGL(k_zero_words, dim3((n + 255) / 256), dim3(256), 0, s, static_cast<uint32_t*>(p), n);
 }
 
+int exclusive_scan_u32(uint32_t* a, uint64_t m, uint32_t* part, hipStream_t s) {
+  if (m == 0) return BRILLE_OK;
+  const uint64_t np = (m + SCAN_SEG - 1) / SCAN_SEG;
+  if (np > 256ull * 1024) return fail(BRILLE_ERR_UNSUPPORTED, "scan too long");
+  hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)np), dim3(BIN_BLOCK), 0, s, a, m, part);
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(BIN_BLOCK), 0, s, part, (uint32_t)np);
+  hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)np), dim3(BIN_BLOCK), 0, s, a, m, part);
+  if (hipGetLastError() != hipSuccess) return fail(BRILLE_ERR_DEVICE, "scan launch failed");
+  return BRILLE_OK;
+}
+
 static void free_work_space(brille_grid* G) {
   if (G->bin_ws) (void)hipFree(G->bin_ws);
   G->bin_ws = nullptr;
@@ -749,12 +795,12 @@ static int ensure_side_stream(brille_grid* G) {
 struct PassBuffers {
   FoldRec* rec;
   SortRec* srt;
-  uint32_t *hist, *part, *plist, *phint, *pcount, *perm, *snode;
+  uint32_t *hist, *part, *plist, *pcount, *perm, *snode;
 };
 
 // A call's passes (BIN_CHUNK points each) run as a two-stage pipeline when
-// there are several: the preparation of pass k + 1 (fold, scans, proposal,
-// scatter, node order) on the grid's side stream while the sums of pass k run
+// there are several: the preparation of pass k + 1 (fold, scans, scatter,
+// node order, proposal) on the grid's side stream while the sums of pass k run
 // on the call's stream, each pass with its own half of the work space.  The
 // fold is bound by FP64 issue and the sums by their scattered stores, and one
 // fold wave per SIMD fits beside four sums waves, so the two overlap.  The
@@ -763,14 +809,14 @@ struct PassBuffers {
 // call stream held before the call: the call is ordered on `s` like a single
 // kernel.
 //
-// stage_ms (nullable): adds the milliseconds of fold, bucket (scans, proposal,
-// scatter, node order) and sums over all passes, each from events on the
+// stage_ms (nullable): adds the milliseconds of fold, bucket (scans, scatter,
+// node order, proposal) and sums over all passes, each from events on the
 // stream that ran it (with several passes the stages overlap, so they add up
 // to more than the call)
 // Byte offsets of one pass's buffers (a half of the work space) for passes of
 // `chunk` points.
 struct PassLayout {
-  uint64_t rec, srt, hist, part, list, hint, perm, node, cnt, half;
+  uint64_t rec, srt, hist, part, list, perm, node, cnt, half;
   uint64_t scan_blocks;  // segments of the histogram scan (k_scan_top handles at most 256 Ki)
 };
 static PassLayout pass_layout(uint64_t chunk, uint32_t nb) {
@@ -784,8 +830,7 @@ static PassLayout pass_layout(uint64_t chunk, uint32_t nb) {
   L.hist = L.srt + al(chunk * sizeof(SortRec));
   L.part = L.hist + al(m_max * 4);
   L.list = L.part + al(np_max * 4);
-  L.hint = L.list + al(chunk * 4);
-  L.perm = L.hint + al(chunk * 4);
+  L.perm = L.list + al(chunk * 4);
   L.node = L.perm + al(chunk * 4);
   L.cnt = L.node + al(chunk * 4);
   L.half = L.cnt + 256;
@@ -829,8 +874,9 @@ int run_binned_passes(brille_grid* G, const LaunchArgs& a, int precision, hipStr
   const uint32_t nb = bucket_count(G);
   const PassLayout Lo = pass_layout(chunk, nb);
   if (Lo.scan_blocks > 256ull * 1024) return fail(BRILLE_ERR_INVALID_ARGUMENT, "binned pass too large");
+  if (G->g.n_nodes >= kNodePoly) return fail(BRILLE_ERR_INVALID_ARGUMENT, "the binned kernel takes grids of fewer than 2^31 nodes");
   const uint64_t o_rec = Lo.rec, o_srt = Lo.srt, o_hist = Lo.hist, o_part = Lo.part, o_list = Lo.list,
-                 o_hint = Lo.hint, o_perm = Lo.perm, o_node = Lo.node, o_cnt = Lo.cnt, half = Lo.half;
+                 o_perm = Lo.perm, o_node = Lo.node, o_cnt = Lo.cnt, half = Lo.half;
   const bool prop = a.use_proposal != 0;
   PassBuffers buf[2];
   for (int h = 0; h < halves; ++h) {
@@ -840,8 +886,7 @@ int run_binned_passes(brille_grid* G, const LaunchArgs& a, int precision, hipStr
     b.srt = reinterpret_cast<SortRec*>(ws + o_srt);
     b.hist = reinterpret_cast<uint32_t*>(ws + o_hist);
     b.part = reinterpret_cast<uint32_t*>(ws + o_part);
-    b.plist = prop ? reinterpret_cast<uint32_t*>(ws + o_list) : nullptr;
-    b.phint = prop ? reinterpret_cast<uint32_t*>(ws + o_hint) : nullptr;
+    b.plist = reinterpret_cast<uint32_t*>(ws + o_list);
     b.pcount = reinterpret_cast<uint32_t*>(ws + o_cnt);
     b.perm = reinterpret_cast<uint32_t*>(ws + o_perm);
     b.snode = reinterpret_cast<uint32_t*>(ws + o_node);
@@ -888,18 +933,21 @@ int run_binned_passes(brille_grid* G, const LaunchArgs& a, int precision, hipStr
     if (prop) zero_words(b.pcount, 1, side);
     mark(k, 0, side);
     hipLaunchKernelGGL(k_bin_fold<true>, dim3(nblk), dim3(BIN_BLOCK), lds1, side, G->g, a.Q + 3 * c0, m, a.fold,
-                       b.rec, b.hist, nblk, nb, a.err, a.base + c0, b.plist, b.pcount);
+                       b.rec, b.hist, nblk, nb, a.err, a.base + c0, prop ? 1 : 0);
     mark(k, 1, side);
     hipLaunchKernelGGL(k_scan_reduce, dim3(np), dim3(BIN_BLOCK), 0, side, b.hist, M, b.part);
     hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(BIN_BLOCK), 0, side, b.part, np);
     hipLaunchKernelGGL(k_scan_apply, dim3(np), dim3(BIN_BLOCK), 0, side, b.hist, M, b.part);
-    if (prop)
-      hipLaunchKernelGGL(k_bin_propose, dim3(std::min<uint32_t>((m + 63) / 64, std::max<uint32_t>(G->n_cu, 1) * 4)),
-                         dim3(256), 0, side, G->g, a.prop, b.rec, b.plist, b.pcount, b.phint);
     hipLaunchKernelGGL(k_bin_scatter, dim3(nblk), dim3(BIN_BLOCK), 2 * lds1, side, b.rec, m, b.hist, nblk, nb,
-                       G->g.n_nodes, b.srt, b.snode, b.phint);
+                       G->g.n_nodes, b.srt, b.snode);
     hipLaunchKernelGGL(k_bucket_perm, dim3(nb + 1), dim3(PERM_BLOCK), 0, side, b.snode, m, b.hist, nblk, nb,
                        G->g.n_nodes, b.perm);
+    if (prop) {
+      hipLaunchKernelGGL(k_poly_list, dim3((m + LIST_SPAN - 1) / LIST_SPAN), dim3(BIN_BLOCK), 0, side, b.perm, m,
+                         b.plist, b.pcount);
+      hipLaunchKernelGGL(k_bin_propose, dim3(std::min<uint32_t>((m + 63) / 64, std::max<uint32_t>(G->n_cu, 1) * 4)),
+                         dim3(256), 0, side, G->g, a.prop, b.srt, b.plist, b.pcount);
+    }
     mark(k, 2, side);
     if (halves > 1) {
       (void)hipEventRecord(G->bin_ev[EV_PREP + h], side);

@@ -317,6 +317,7 @@ int brille_grid_destroy(brille_grid* G) {
   (void)hipDeviceSynchronize();
   free_all(G->grid_allocs);
   free_all(G->data_allocs);
+  free_all(G->sort_allocs);
   free_proposal(G);
   free_staging(G);
   free_binned(G);
@@ -431,6 +432,8 @@ static int grid_fill(brille_grid* G, const brille_data_desc* vals, const brille_
   }
   (void)hipDeviceSynchronize();  // nothing still reads the payload that is replaced
   free_all(G->data_allocs);
+  free_all(G->sort_allocs);
+  G->n_perms = 0;
   ++G->fill_gen;  // the payload tables are replaced from here on (batches re-read them)
   G->filled = false;
   DataConst& d = G->d;
@@ -463,6 +466,7 @@ static int grid_fill(brille_grid* G, const brille_data_desc* vals, const brille_
     chk(upload(perm->keys, (size_t)perm->n_keys, &d.keys, A));
     chk(upload(perm->perm_index, (size_t)perm->n_keys, &d.perm_index, A));
     chk(upload(perm->perms, (size_t)perm->n_perms * d.B, &d.perms, A));
+    G->n_perms = perm->n_perms;
   }
   if (rc != BRILLE_OK) {
     free_all(A);

@@ -25,6 +25,9 @@ struct brille_grid {
   brille_amd::DataConst d{};
   std::vector<void*> grid_allocs;
   std::vector<void*> data_allocs;
+  // the permutation table brille_grid_sort built (freed by the next sort or fill)
+  std::vector<void*> sort_allocs;
+  uint32_t n_perms = 0;  // permutations of the installed table
   bool filled = false;
   uint64_t n_vertices = 0, n_cubes = 0, n_polys = 0, n_tets = 0;
   brille_amd::ErrRec* err = nullptr;
@@ -102,6 +105,9 @@ void free_binned(brille_grid* G);
 // clears n 32-bit words on stream s with a kernel (no memset node, so that a
 // caller may capture the call in a graph)
 void zero_words(void* p, uint32_t n, hipStream_t s);
+// exclusive prefix sum of a[0, m) in place on stream s; part holds
+// (m + 4095) / 4096 words; m at most 2^30
+int exclusive_scan_u32(uint32_t* a, uint64_t m, uint32_t* part, hipStream_t s);
 // the cube row table of the bucketed sums, built on first use (data_allocs)
 int ensure_cube_rows(brille_grid* G, int precision, hipStream_t s);
 // the first failing point of a call as the reference's error text

@@ -230,7 +230,7 @@ BZTrellisQdc::BZTrellisQdc(const BrillouinZone& b, double node_volume, bool alwa
   zone.build(bz);
 }
 
-void BZTrellisQdc::fill(Payload v, Payload c) {
+void BZTrellisQdc::fill(Payload v, Payload c, bool sort) {
   std::lock_guard<std::mutex> lock(*host_mu);
   const size_t V = trellis.n_vertices();
   if (v.shape[0] != V || c.shape[0] != V)
@@ -241,6 +241,7 @@ void BZTrellisQdc::fill(Payload v, Payload c) {
   vectors = std::move(c);
   filled = true;
   device_filled = false;
+  sort_requested = sort;
   if (vectors.rot == BRILLE_ROT_GAMMA)
     gamma.construct(bz.outer, bz.psym, bz.time_reversal, approx.direct, approx.digits);
 }
@@ -306,6 +307,8 @@ brille_grid* BZTrellisQdc::handle() {
       gp = &g;
     }
     check_status(brille_grid_fill(device->h, &dv, &dc, gp, nullptr));
+    // wrap/_common_grid.hpp:39-43: sort after the data are in place
+    if (sort_requested) check_status(brille_grid_sort(device->h, &values.cost, &vectors.cost));
     device_filled = true;
   }
   return device->h;

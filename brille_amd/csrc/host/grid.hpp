@@ -50,6 +50,10 @@ struct Payload {
   int rot = BRILLE_ROT_VECTOR;
   int lu = BRILLE_LU_REAL_LATTICE;
   uint32_t branches = 0;
+  // mode-assignment cost (Interpolator::set_cost_info and _costmult,
+  // src/interpolator.hpp:246-305); the reference's defaults unless a
+  // weighted fill gave others
+  brille_cost_info cost{0, 0, {1.0, 1.0, 1.0}};
   uint32_t span() const { return elements[0] + elements[1] + elements[2]; }
 };
 
@@ -67,13 +71,15 @@ class BZTrellisQdc {
   ZoneTables zone;
   std::shared_ptr<DeviceGrid> device;
   bool device_filled = false;
+  // fill(..., sort=True): the device table is sorted right after the device fill
+  bool sort_requested = false;
   // guards the lazy device create + fill and the host payload it reads:
   // callers release the GIL around interpolation, so several threads can
   // reach handle() at once
   std::shared_ptr<std::mutex> host_mu = std::make_shared<std::mutex>();
 
   BZTrellisQdc(const BrillouinZone& bz, double node_volume_fraction, bool always_triangulate, ApproxConfig cfg);
-  void fill(Payload v, Payload c);
+  void fill(Payload v, Payload c, bool sort = false);
   brille_grid_desc describe() const;
   brille_grid* handle();  // create + fill on first use
   uint64_t bytes_per_point(int precision = BRILLE_FP64) const;

@@ -97,6 +97,14 @@ Payload payload_from(const py::array& arr, const iarr& el, bool complex_data) {
   return p;
 }
 
+// the cost functions and weights of a weighted fill  wrap/_interpolator.hpp:96-105
+void set_cost(Payload& p, const iarr& el, const darr& w) {
+  if (w.ndim() != 1) throw std::runtime_error("weights must be a 1-D array");
+  if (el.size() > 5) p.cost.scalar_cost = el.data()[5];
+  if (el.size() > 6) p.cost.vector_cost = el.data()[6];
+  for (py::ssize_t i = 0; i < w.size() && i < 3; ++i) p.cost.weights[i] = w.data()[i];
+}
+
 py::tuple grid_interpolate(PyGrid& pg, const darr& Q, bool useparallel, int threads, bool no_move,
                            const std::string& precision, const std::string& kernel, bool first_zone) {
   (void)useparallel;
@@ -460,22 +468,41 @@ PYBIND11_MODULE(_brille, m) {
       .def_property_readonly("bytes_per_point", [](const PyGrid& g) { return g.g->bytes_per_point(); })
       .def_property_readonly("values", [](const PyGrid& g) { return g.values_ref; })
       .def_property_readonly("vectors", [](const PyGrid& g) { return g.vectors_ref; })
+      // wrap/_common_grid.hpp:26-47: sort=True sorts the modes of every
+      // connected vertex pair once the data are on the device
       .def("fill", [](PyGrid& g, const py::array& vals, const iarr& vel, const py::array& vecs, const iarr& cel, bool sort) {
-        if (sort) throw std::runtime_error("sort=True (mode permutation assignment) is not supported by this build");
-        g.g->fill(payload_from(vals, vel, false), payload_from(vecs, cel, true));
+        g.g->fill(payload_from(vals, vel, false), payload_from(vecs, cel, true), sort);
         g.values_ref = vals;
         g.vectors_ref = vecs;
       }, "values_data"_a, "values_elements"_a, "vectors_data"_a, "vectors_elements"_a, "sort"_a = false)
+      // wrap/_common_grid.hpp:126-147 with wrap/_interpolator.hpp:56-107:
+      // elements[5] and [6] pick the scalar and vector cost functions, the
+      // weights scale the scalar, vector and matrix terms
       .def("fill", [](PyGrid& g, const py::array& vals, const iarr& vel, const darr& vw, const py::array& vecs,
                       const iarr& cel, const darr& cw, bool sort) {
-        (void)vw;
-        (void)cw;
-        if (sort) throw std::runtime_error("sort=True (mode permutation assignment) is not supported by this build");
-        g.g->fill(payload_from(vals, vel, false), payload_from(vecs, cel, true));
+        Payload pv = payload_from(vals, vel, false), pc = payload_from(vecs, cel, true);
+        set_cost(pv, vel, vw);
+        set_cost(pc, cel, cw);
+        g.g->fill(std::move(pv), std::move(pc), sort);
         g.values_ref = vals;
         g.vectors_ref = vecs;
       }, "values_data"_a, "values_elements"_a, "values_weights"_a, "vectors_data"_a, "vectors_elements"_a,
          "vectors_weights"_a, "sort"_a = false)
+      .def("sort", [](PyGrid& g) {
+        brille_grid* h = g.g->handle();
+        check_status(brille_grid_sort(h, &g.g->values.cost, &g.g->vectors.cost));
+      })
+      .def("permutation_table", [](PyGrid& g) {
+        brille_grid* h = g.g->handle();
+        uint64_t nk = 0;
+        uint32_t np = 0;
+        check_status(brille_grid_permutations(h, &nk, &np, nullptr, nullptr, nullptr));
+        py::array_t<uint64_t> keys((py::ssize_t)nk);
+        py::array_t<uint32_t> index((py::ssize_t)nk);
+        py::array_t<uint32_t> perms({(py::ssize_t)np, (py::ssize_t)g.g->values.branches});
+        check_status(brille_grid_permutations(h, &nk, &np, keys.mutable_data(), index.mutable_data(), perms.mutable_data()));
+        return py::make_tuple(keys, index, perms);
+      })
       .def(
           "ir_interpolate_at",
           [](PyGrid& pg, const darr& Q, bool par, int th, bool nm, const std::string& prec, const std::string& k) {

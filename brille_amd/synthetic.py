@@ -306,3 +306,34 @@ def batch_grids(n_grids=64, v_range=(2e3, 2e5), n_q=10_000_000, box=3.0, cfg=2, 
                     f"(target {spec['v_target']:.0f})")
     Q, offsets = batch_queries(n_grids, n_q, box, cfg)
     return grids, Q, offsets
+
+
+def shuffled_modes(tables, seed=0, spread=10.0):
+    """A payload whose modes never cross, then shuffled independently at every
+    vertex, for the mode sort.
+
+    Mode b has eigenvalue spread * b + a_b . x (x Cartesian, |a_b| < 1 over
+    the zone) and eigenvector phase_v,b (u_b + 0.05 g_b . x), with unit u_b
+    drawn at random (nearly orthogonal), a random phase per vertex and mode,
+    and the element layout of `tables`.  Returns (values, vectors, order):
+    the shuffled payload and order[v, a], the original mode stored at vertex
+    v as mode a; the sort of an exact assignment maps mode a of vertex i to
+    the mode b of vertex j with order[j, b] == order[i, a]."""
+    r = np.random.default_rng(np.random.SeedSequence([2026, 99, int(seed)]))
+    x = np.asarray(tables["vertices"], dtype=np.float64)
+    V = x.shape[0]
+    B = int(tables["branches"])
+    sv = int(np.sum(tables["values_elements"]))
+    sc = int(np.sum(tables["vectors_elements"]))
+    scale = max(1e-12, float(np.max(np.abs(x))))
+    a = r.normal(size=(B, 3)) * 0.3 / scale
+    vals = spread * np.arange(B)[None, :] + x @ a.T
+    values = np.repeat(vals[:, :, None], sv, axis=2)
+    u = r.normal(size=(B, sc)) + 1j * r.normal(size=(B, sc))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    gk = (r.normal(size=(B, sc, 3)) + 1j * r.normal(size=(B, sc, 3))) * 0.05 / scale
+    vec = u[None] + np.einsum("bek,vk->vbe", gk, x)
+    vec *= np.exp(2j * np.pi * r.random((V, B)))[:, :, None]
+    order = np.argsort(r.random((V, B)), axis=1)
+    rows = np.arange(V)[:, None]
+    return np.ascontiguousarray(values[rows, order]), np.ascontiguousarray(vec[rows, order]), order

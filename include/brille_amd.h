@@ -57,7 +57,8 @@ enum brille_status {
   BRILLE_ERR_ROTATION = 7,         /* "LengthUnit, RotatesLike combination not implemented" src/interpolator.hpp:400-426 */
   BRILLE_ERR_DEVICE = 8,           /* a HIP runtime call failed */
   BRILLE_ERR_NO_DEVICE = 9,        /* no usable gfx950 device */
-  BRILLE_ERR_UNSUPPORTED = 10      /* a size beyond the kernels' documented limits */
+  BRILLE_ERR_UNSUPPORTED = 10,     /* a size beyond the kernels' documented limits */
+  BRILLE_ERR_NUMERIC = 11          /* "Complex Hermitian angle cos(theta)=... " src/utilities.tpp:362-378 */
 };
 
 /* Node types, numbered as the reference's NodeType enum (src/enums.hpp:43). */
@@ -251,6 +252,37 @@ typedef struct brille_fill_opts {
 int brille_grid_fill_ex(brille_grid* grid, const brille_data_desc* values, const brille_data_desc* vectors,
                         const brille_gamma_desc* gamma, const brille_perm_desc* perm,
                         const brille_fill_opts* opts);
+/* ---- mode sorting ---- */
+/* The mode-assignment cost of one interpolator (Interpolator::set_cost_info,
+ * src/interpolator.hpp:246-305, and its _costmult): scalar_cost 0 is the sum
+ * of |a - b| over the scalar elements (the only kind); vector_cost 0 (and any
+ * value above 4) is sin^2 of the Hermitian angle, 1 the vector distance, 2
+ * one minus the vector product, 3 the vector angle, 4 the Hermitian angle;
+ * weights scale the scalar, vector-element and matrix-element terms. */
+typedef struct brille_cost_info {
+  int32_t scalar_cost;
+  int32_t vector_cost;
+  double weights[3];
+} brille_cost_info;
+/* DualInterpolator::sort  src/interpolatordual.hpp:398-434 (called by the
+ * wrapper's fill(..., sort=True), wrap/_common_grid.hpp:39-43): for every
+ * pair of vertices that share a cube node or a tetrahedron
+ * (PolyTrellis::collect_keys, src/trellis_poly_parallel.tpp:487-524), the
+ * assignment of the second vertex's modes to the first's that minimises the
+ * summed cost (src/interpolatordual.hpp:384-396, src/interpolator_cost.tpp:21-58),
+ * stored for (i, j) and its inverse for (j, i).  Replaces the grid's
+ * permutation table; needs a filled grid with complex vectors.  NULL cost
+ * infos take the reference's defaults ({0, 0, {1, 1, 1}}).  The costs use the
+ * FP64 payload rows when they were kept, else the FP32 rows. */
+int brille_grid_sort(brille_grid* grid, const brille_cost_info* values_cost, const brille_cost_info* vectors_cost);
+/* The grid's permutation table in brille_perm_desc form (keys sorted, one
+ * permutation per key, index 0 the identity).  Call with NULL arrays to read
+ * n_keys and n_perms, then with arrays of those sizes (perms: n_perms x
+ * branches).  No reference counterpart: the reference keeps the table inside
+ * the grid (src/permutation_table.hpp:63-214). */
+int brille_grid_permutations(const brille_grid* grid, uint64_t* n_keys, uint32_t* n_perms, uint64_t* keys,
+                             uint32_t* perm_index, uint32_t* perms);
+
 /* bytes of interpolation output per Q at a precision  src/interpolatordual.hpp:297-299 */
 uint64_t brille_bytes_per_point(const brille_grid* grid, int32_t precision);
 
@@ -290,8 +322,10 @@ int brille_ir_moveinto(brille_grid* grid, const double* Q, uint64_t n, int32_t m
 /* A batch holds filled grids (not owned; they must outlive it).  A run is one
  * persistent kernel whose blocks take tiles of points from a work counter
  * across all grids, captured once into a HIP graph per precision and schedule
- * and replayed; the tile shape, block count, tiles per grab and grid order
- * come from brille_batch_autotune (defaults before it runs). */
+ * and replayed, followed on the same stream by one binned call per grid
+ * given at least `binned_min` points; the tile shape, block count, tiles per
+ * grab, grid order and binned_min come from brille_batch_autotune (defaults
+ * before it runs: no binned grids). */
 int brille_batch_create(brille_grid* const* grids, uint32_t n_grids, brille_batch** out);
 /* Q: all points, grid g owns rows [offsets[g], offsets[g+1]); values_out[g] and
  * vectors_out[g] receive that grid's results.  Device memory only.  kernel =
@@ -305,9 +339,16 @@ int brille_batch_autotune(brille_batch* batch, const double* Q, const uint64_t* 
                           const brille_run_opts* opts, void* const* values_out,
                           void* const* vectors_out, void* stream);
 /* the schedule in use: points per tile (= threads per block), persistent
- * blocks, tiles per grab, grid order (0 as given, 1 costliest first) */
+ * blocks, tiles per grab, grid order (0 as given, 1 costliest first), and
+ * the point count from which a grid runs its own binned pipeline after the
+ * persistent kernel instead (UINT64_MAX: none) */
 int brille_batch_schedule(const brille_batch* batch, uint32_t* tile_points, uint32_t* blocks,
-                          uint32_t* tiles_per_grab, uint32_t* order);
+                          uint32_t* tiles_per_grab, uint32_t* order, uint64_t* binned_min);
+/* sets the schedule (a saved autotune result, or a chosen one): tile_points
+ * 128 or 256, blocks per CU and tiles per grab 1-64, order 0 or 1,
+ * binned_min as above */
+int brille_batch_set_schedule(brille_batch* batch, uint32_t tile_points, uint32_t blocks_per_cu,
+                              uint32_t tiles_per_grab, uint32_t order, uint64_t binned_min);
 int brille_batch_destroy(brille_batch* batch);
 
 /* ---- learned tetrahedron proposal (no reference counterpart) ---- */

@@ -860,3 +860,321 @@ uint32_t brille_oracle_find_bin(const double* edges, uint32_t n, double x) { ret
 void brille_oracle_antiphase(uint32_t n, const double* a, const double* b, double* out) {
   antiphase(n, a, b, out, out + 1);
 }
+
+/* ------------------------------------------------------------------------ */
+/* mode sorting: DualInterpolator::sort  src/interpolatordual.hpp:398-434    */
+/* ------------------------------------------------------------------------ */
+/* approx_float::scalar(act, 1.0) with its default tolerances, then the
+ * clamp of |cos| to one (src/utilities.tpp:263-275, 369-373); returns 0 when
+ * |cos| exceeds one by more than that (the reference throws there). */
+static int clamp_cos(double* c) {
+  double act = fabs(*c);
+  if (approx(act, 1.0, make_tols(0.0, 1)) && act > 1) {
+    *c /= act;
+    act = fabs(*c);
+  }
+  return act <= 1;
+}
+
+/* utils::vector_angle, real  src/utilities.tpp:249-278 */
+static double vec_angle_r(uint32_t n, const double* A, const double* B, int* ok) {
+  double AA = 0, BB = 0, AB = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    AA += A[i] * A[i];
+    BB += B[i] * B[i];
+    AB += A[i] * B[i];
+  }
+  double nA = sqrt(AA), nB = sqrt(BB), c;
+  if (nA != 0 && nB != 0) c = AB / (nA * nB);
+  else c = (nA != 0 || nB != 0) ? 0.0 : 1.0;
+  if (!clamp_cos(&c)) *ok = 0;
+  return acos(c);
+}
+
+/* utils::euclidean_angle (vector_angle of complex vectors)  src/utilities.tpp:280-312 */
+static double vec_angle_c(uint32_t n, const double* A, const double* B, int* ok) {
+  double AB = 0, nA = 0, nB = 0, c;
+  for (uint32_t i = 0; i < n; ++i) {
+    AB += A[2 * i] * B[2 * i] + A[2 * i + 1] * B[2 * i + 1];
+    nA += A[2 * i] * A[2 * i] + A[2 * i + 1] * A[2 * i + 1];
+    nB += B[2 * i] * B[2 * i] + B[2 * i + 1] * B[2 * i + 1];
+  }
+  nA = sqrt(nA);
+  nB = sqrt(nB);
+  if (nA != 0 && nB != 0) c = AB / (nA * nB);
+  else c = (nA != 0 || nB != 0) ? 0.0 : 1.0;
+  if (!clamp_cos(&c)) *ok = 0;
+  return acos(c);
+}
+
+/* utils::hermitian_product of complex vectors, sum of conj(a) b in element
+ * order  src/utilities.tpp:340-350 */
+static void herm_prod_c(uint32_t n, const double* a, const double* b, double* re, double* im) {
+  double hr = 0, hi = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    const double ar = a[2 * i], ai = a[2 * i + 1], br = b[2 * i], bi = b[2 * i + 1];
+    hr += ar * br + ai * bi;
+    hi += ar * bi - ai * br;
+  }
+  *re = hr;
+  *im = hi;
+}
+
+/* utils::vector_product, complex: |a^H b|^2  src/utilities.tpp:414-417 */
+static double vec_prod_c(uint32_t n, const double* a, const double* b) {
+  double hr, hi;
+  herm_prod_c(n, a, b, &hr, &hi);
+  return hr * hr + hi * hi;  /* real(h * conj(h)) */
+}
+
+/* utils::hermitian_angle, complex  src/utilities.tpp:357-381 */
+static double herm_angle_c(uint32_t n, const double* A, const double* B, int* ok) {
+  double nAB = sqrt(vec_prod_c(n, A, B)), hr, hi;
+  herm_prod_c(n, A, A, &hr, &hi);
+  double nA = sqrt(hr);
+  herm_prod_c(n, B, B, &hr, &hi);
+  double nB = sqrt(hr), c;
+  if (nA != 0 && nB != 0) c = nAB / (nA * nB);
+  else c = (nA != 0 || nB != 0) ? 0.0 : 1.0;
+  if (!clamp_cos(&c)) *ok = 0;
+  return acos(c);
+}
+
+/* utils::vector_distance  src/utilities.tpp:383-399 */
+static double vec_dist_r(uint32_t n, const double* a, const double* b) {
+  double s = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    double d = a[i] - b[i];
+    s += d * d;
+  }
+  return sqrt(s);
+}
+static double vec_dist_c(uint32_t n, const double* a, const double* b) {
+  double s = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    double dr = a[2 * i] - b[2 * i], di = a[2 * i + 1] - b[2 * i + 1];
+    s += dr * dr + di * di;  /* real(d * conj(d)) */
+  }
+  return sqrt(s);
+}
+
+/* the vector cost functions  src/interpolator.hpp:256-293 */
+static double vecfun_r(int t, uint32_t n, const double* a, const double* b, int* ok) {
+  switch (t) {
+    case 1: return vec_dist_r(n, a, b);
+    case 2: {
+      double h = 0;
+      for (uint32_t i = 0; i < n; ++i) h += a[i] * b[i];
+      return 1 - h;
+    }
+    case 3: return vec_angle_r(n, a, b, ok);
+    case 4: return vec_angle_r(n, a, b, ok); /* hermitian_angle of real vectors */
+    default: {
+      double s = sin(vec_angle_r(n, a, b, ok));
+      return s * s;
+    }
+  }
+}
+static double vecfun_c(int t, uint32_t n, const double* a, const double* b, int* ok) {
+  switch (t) {
+    case 1: return vec_dist_c(n, a, b);
+    case 2: return 1 - vec_prod_c(n, a, b);
+    case 3: return vec_angle_c(n, a, b, ok);
+    case 4: return herm_angle_c(n, a, b, ok);
+    default: {
+      double s = sin(herm_angle_c(n, a, b, ok));
+      return s * s;
+    }
+  }
+}
+
+/* Interpolator::add_cost for the real values, no phase freedom
+ * src/interpolator_cost.tpp:42-57 */
+static void add_cost_values(uint32_t B, const uint32_t* e, const double* x0, const double* x1,
+                            const brille_cost_info* ci, double* cost, int* ok) {
+  const uint32_t s_ = e[0] + e[1] + e[2], mo = e[0] + e[1];
+  for (uint32_t i = 0; i < B; ++i) {
+    const double* a = x0 + (size_t)i * s_;
+    for (uint32_t j = 0; j < B; ++j) {
+      const double* b = x1 + (size_t)j * s_;
+      double s_cost = 0, v_cost = 0, m_cost = 0;
+      if (e[0]) {
+        s_cost = 0;
+        for (uint32_t z = 0; z < e[0]; ++z) s_cost += fabs(a[z] - b[z]);
+      }
+      if (e[1]) v_cost = vecfun_r(ci->vector_cost, e[1], a + e[0], b + e[0], ok);
+      if (e[2]) {
+        m_cost = 0;
+        for (uint32_t m = 0; m < e[2] / 9; ++m) m_cost += vec_dist_r(9, a + mo + 9 * m, b + mo + 9 * m);
+      }
+      cost[i * B + j] += ci->weights[0] * s_cost + ci->weights[1] * v_cost + ci->weights[2] * m_cost;
+    }
+  }
+}
+
+/* Interpolator::add_cost for the complex vectors, each pair compared after
+ * the second is multiplied by its antiphase  src/interpolator_cost.tpp:26-41 */
+static void add_cost_vectors(uint32_t B, const uint32_t* e, const double* x0, const double* x1,
+                             const brille_cost_info* ci, double* cost, double* phased, int* ok) {
+  const uint32_t s_ = e[0] + e[1] + e[2], mo = e[0] + e[1];
+  for (uint32_t i = 0; i < B; ++i) {
+    const double* a = x0 + 2 * (size_t)i * s_;
+    for (uint32_t j = 0; j < B; ++j) {
+      const double* b = x1 + 2 * (size_t)j * s_;
+      double er, ei;
+      antiphase(s_, a, b, &er, &ei);
+      for (uint32_t z = 0; z < s_; ++z) { /* inplace_antiphase  src/utilities.tpp:589-593 */
+        phased[2 * z] = er * b[2 * z] - ei * b[2 * z + 1];
+        phased[2 * z + 1] = er * b[2 * z + 1] + ei * b[2 * z];
+      }
+      double s_cost = 0, v_cost = 0, m_cost = 0;
+      if (e[0]) {
+        s_cost = 0;
+        for (uint32_t z = 0; z < e[0]; ++z) {
+          double dr = a[2 * z] - phased[2 * z], di = a[2 * z + 1] - phased[2 * z + 1];
+          s_cost += sqrt(dr * dr + di * di); /* utils::magnitude of a complex difference */
+        }
+      }
+      if (e[1]) v_cost = vecfun_c(ci->vector_cost, e[1], a + 2 * e[0], phased + 2 * e[0], ok);
+      if (e[2]) {
+        m_cost = 0;
+        for (uint32_t m = 0; m < e[2] / 9; ++m) m_cost += vec_dist_c(9, a + 2 * (mo + 9 * m), phased + 2 * (mo + 9 * m));
+      }
+      cost[i * B + j] += ci->weights[0] * s_cost + ci->weights[1] * v_cost + ci->weights[2] * m_cost;
+    }
+  }
+}
+
+/* Linear assignment: the n x n cost matrix C (row-major) is solved exactly
+ * by successive shortest augmenting paths with row and column potentials
+ * (Jonker and Volgenant, Computing 38, 325 (1987); the O(n^3) Hungarian
+ * form of Kuhn and Munkres).  Rows are added in order 0..n-1; in each
+ * Dijkstra step the unscanned column of least reduced distance is taken,
+ * the lowest column index on ties.  row[i] = column of row i, col[j] = row
+ * of column j.  For a cost matrix with a unique optimum every exact solver,
+ * the reference's lapjv (src/lapjv.hpp) included, returns that optimum. */
+static void lap_solve(uint32_t n, const double* C, int32_t* row, int32_t* col, double* u, double* v, double* minv,
+                      int32_t* p, int32_t* way, uint8_t* used) {
+  for (uint32_t j = 0; j <= n; ++j) {
+    u[j] = 0;
+    v[j] = 0;
+    p[j] = 0;
+    way[j] = 0;
+  }
+  for (uint32_t i = 1; i <= n; ++i) {
+    p[0] = (int32_t)i;
+    uint32_t j0 = 0;
+    for (uint32_t j = 0; j <= n; ++j) {
+      minv[j] = INFINITY;
+      used[j] = 0;
+    }
+    do {
+      used[j0] = 1;
+      const uint32_t i0 = (uint32_t)p[j0];
+      double delta = INFINITY;
+      uint32_t j1 = 0;
+      for (uint32_t j = 1; j <= n; ++j) {
+        if (used[j]) continue;
+        const double cur = C[(size_t)(i0 - 1) * n + (j - 1)] - u[i0] - v[j];
+        if (cur < minv[j]) {
+          minv[j] = cur;
+          way[j] = (int32_t)j0;
+        }
+        if (minv[j] < delta) {
+          delta = minv[j];
+          j1 = j;
+        }
+      }
+      for (uint32_t j = 0; j <= n; ++j) {
+        if (used[j]) {
+          u[p[j]] += delta;
+          v[j] -= delta;
+        } else {
+          minv[j] -= delta;
+        }
+      }
+      if (j1 == 0) return; /* no finite reduced cost left (NaN costs) */
+      j0 = j1;
+    } while (p[j0] != 0);
+    do {
+      const uint32_t j1 = (uint32_t)way[j0];
+      p[j0] = p[j1];
+      j0 = j1;
+    } while (j0);
+  }
+  for (uint32_t j = 1; j <= n; ++j) {
+    row[p[j] - 1] = (int32_t)(j - 1);
+    col[j - 1] = p[j] - 1;
+  }
+}
+
+int brille_oracle_lap(uint32_t n, const double* C, int32_t* row, int32_t* col) {
+  double* buf = (double*)malloc(3 * (size_t)(n + 1) * sizeof(double));
+  int32_t* ib = (int32_t*)malloc(2 * (size_t)(n + 1) * sizeof(int32_t));
+  uint8_t* used = (uint8_t*)malloc(n + 1);
+  if (!buf || !ib || !used) {
+    free(buf);
+    free(ib);
+    free(used);
+    return BRILLE_ERR_INVALID_ARGUMENT;
+  }
+  lap_solve(n, C, row, col, buf, buf + n + 1, buf + 2 * (n + 1), ib, ib + n + 1, used);
+  free(buf);
+  free(ib);
+  free(used);
+  return BRILLE_OK;
+}
+
+/* DualInterpolator::cost_matrix + determine_permutation_ij for the listed
+ * pairs (i, j), i != j  src/interpolatordual.hpp:384-396, 422-432:
+ * values->data is V x B x sv doubles, vectors->data V x B x sc complex.  For
+ * pair k, row[k*B + a] is the mode of vertex j assigned to mode a of vertex
+ * i (stored for key (i, j)), col[k*B + b] its inverse (key (j, i));
+ * cost_out (nullable) receives the B x B matrices.  OpenMP over pairs as the
+ * reference (src/interpolatordual.hpp:415-420). */
+int brille_oracle_sort_pairs(uint64_t n_pairs, const uint32_t* pairs, const brille_data_desc* values,
+                             const brille_cost_info* vcost, const brille_data_desc* vectors,
+                             const brille_cost_info* ccost, int32_t* row, int32_t* col, double* cost_out,
+                             int nthreads) {
+  const uint32_t B = values->branches;
+  if (vectors->branches != B || !vectors->is_complex || values->is_complex) return BRILLE_ERR_INVALID_ARGUMENT;
+  const uint32_t sv = values->elements[0] + values->elements[1] + values->elements[2];
+  const uint32_t sc = vectors->elements[0] + vectors->elements[1] + vectors->elements[2];
+  int result = BRILLE_OK;
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+  (void)nthreads;
+#endif
+#pragma omp parallel
+  {
+    double* cost = (double*)malloc((size_t)B * B * sizeof(double));
+    double* phased = (double*)malloc(2 * (size_t)(sc ? sc : 1) * sizeof(double));
+    double* buf = (double*)malloc(3 * (size_t)(B + 1) * sizeof(double));
+    int32_t* ib = (int32_t*)malloc(2 * (size_t)(B + 1) * sizeof(int32_t));
+    uint8_t* used = (uint8_t*)malloc(B + 1);
+#pragma omp for schedule(dynamic, 64)
+    for (int64_t k = 0; k < (int64_t)n_pairs; ++k) {
+      const uint32_t i = pairs[2 * k], j = pairs[2 * k + 1];
+      int ok = 1;
+      for (uint32_t z = 0; z < B * B; ++z) cost[z] = 0;
+      if (sv) add_cost_values(B, values->elements, values->data + (size_t)i * B * sv, values->data + (size_t)j * B * sv,
+                              vcost, cost, &ok);
+      if (sc) add_cost_vectors(B, vectors->elements, vectors->data + 2 * (size_t)i * B * sc,
+                               vectors->data + 2 * (size_t)j * B * sc, ccost, cost, phased, &ok);
+      if (!ok) {
+#pragma omp critical
+        result = BRILLE_ERR_NUMERIC;
+      }
+      if (cost_out) memcpy(cost_out + (size_t)k * B * B, cost, (size_t)B * B * sizeof(double));
+      lap_solve(B, cost, row + (size_t)k * B, col + (size_t)k * B, buf, buf + B + 1, buf + 2 * (B + 1), ib,
+                ib + B + 1, used);
+    }
+    free(cost);
+    free(phased);
+    free(buf);
+    free(ib);
+    free(used);
+  }
+  return result;
+}

@@ -8,7 +8,7 @@ import os
 
 import numpy as np
 
-from brille_amd.capi import AuxOut, DataHolder, GridDesc, DataDesc, GammaDesc, PermDesc, TableHolder
+from brille_amd.capi import AuxOut, CostInfo, DataHolder, GridDesc, DataDesc, GammaDesc, PermDesc, TableHolder, cost_info
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
@@ -45,6 +45,12 @@ def lib():
         L.brille_oracle_find_bin.argtypes = [_dp, C.c_uint32, C.c_double]
         L.brille_oracle_stage_seconds.restype = None
         L.brille_oracle_stage_seconds.argtypes = [_dp]
+        L.brille_oracle_lap.restype = C.c_int
+        L.brille_oracle_lap.argtypes = [C.c_uint32, _dp, C.c_void_p, C.c_void_p]
+        L.brille_oracle_sort_pairs.restype = C.c_int
+        L.brille_oracle_sort_pairs.argtypes = [C.c_uint64, C.c_void_p, C.POINTER(DataDesc), C.POINTER(CostInfo),
+                                               C.POINTER(DataDesc), C.POINTER(CostInfo), C.c_void_p, C.c_void_p,
+                                               C.c_void_p, C.c_int]
         L.brille_oracle_antiphase.restype = None
         L.brille_oracle_antiphase.argtypes = [C.c_uint32, _dp, _dp, _dp]
         _lib = L
@@ -166,3 +172,77 @@ def antiphase(a, b):
     out = np.zeros(2)
     lib().brille_oracle_antiphase(a.shape[0], a.ctypes.data_as(_dp), b.ctypes.data_as(_dp), out.ctypes.data_as(_dp))
     return complex(out[0], out[1])
+
+
+def lap(cost):
+    """Exact linear assignment of a square cost matrix: (row, col) with
+    row[i] the column of row i and col[j] the row of column j."""
+    c = _d(cost)
+    n = c.shape[0]
+    row = np.zeros(n, np.int32)
+    col = np.zeros(n, np.int32)
+    rc = lib().brille_oracle_lap(n, c.ctypes.data_as(_dp), row.ctypes.data, col.ctypes.data)
+    assert rc == 0
+    return row, col
+
+
+def sort_keys(tables):
+    """The permutation-table keys of a trellis, i*V + j for every ordered pair
+    of distinct vertices that share a cube node or a tetrahedron
+    (PolyTrellis::collect_keys, src/trellis_poly_parallel.tpp:487-524;
+    permutation_table_keys_from_indicies, src/permutation_table.hpp:284-292),
+    sorted."""
+    V = np.uint64(np.asarray(tables["vertices"]).shape[0])
+    units = []
+    cv = np.asarray(tables["cube_vertices"], dtype=np.uint64).reshape(-1, 8)
+    if cv.size:
+        units.append(cv)
+    tv = np.asarray(tables["tet_vertices"], dtype=np.uint64).reshape(-1, 4)
+    if tv.size:
+        units.append(tv)
+    keys = []
+    for u in units:
+        k = u.shape[1]
+        for a in range(k):
+            for b in range(k):
+                if a == b:
+                    continue
+                i, j = u[:, a], u[:, b]
+                sel = i != j
+                keys.append(i[sel] * V + j[sel])
+    if not keys:
+        return np.zeros(0, np.uint64)
+    return np.unique(np.concatenate(keys))
+
+
+def sort_pairs(tables, pairs, values=None, vectors=None, values_cost=None, vectors_cost=None, nthreads=8,
+               want_cost=False):
+    """DualInterpolator cost_matrix + LAPJV for the (i, j) rows of `pairs`:
+    (rc, row, col[, cost]) with row[k] the assignment stored for key (i, j) and
+    col[k] the one for (j, i)."""
+    dh = DataHolder(tables, values, vectors)
+    pairs = np.ascontiguousarray(pairs, dtype=np.uint32).reshape(-1, 2)
+    n = pairs.shape[0]
+    B = dh.vdesc.branches
+    row = np.zeros((n, B), np.int32)
+    col = np.zeros((n, B), np.int32)
+    cost = np.zeros((n, B, B)) if want_cost else None
+    vc = values_cost if values_cost is not None else cost_info()
+    cc = vectors_cost if vectors_cost is not None else cost_info()
+    rc = lib().brille_oracle_sort_pairs(n, pairs.ctypes.data, C.byref(dh.vdesc), C.byref(vc), C.byref(dh.cdesc),
+                                        C.byref(cc), row.ctypes.data, col.ctypes.data,
+                                        cost.ctypes.data if want_cost else None, int(nthreads))
+    return (rc, row, col, cost) if want_cost else (rc, row, col)
+
+
+def sorted_table(tables, values=None, vectors=None, values_cost=None, vectors_cost=None, nthreads=8):
+    """The whole sorted permutation table as {key: permutation}, as
+    DualInterpolator::sort leaves it (src/interpolatordual.hpp:398-434)."""
+    keys = sort_keys(tables)
+    V = np.uint64(np.asarray(tables["vertices"]).shape[0])
+    i = (keys // V).astype(np.uint32)
+    j = (keys % V).astype(np.uint32)
+    fwd = i < j
+    pairs = np.stack([i[fwd], j[fwd]], axis=1)
+    rc, row, col = sort_pairs(tables, pairs, values, vectors, values_cost, vectors_cost, nthreads)
+    return rc, keys, pairs, row, col

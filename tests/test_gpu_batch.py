@@ -103,6 +103,27 @@ def test_batch_autotune_and_sequential_baseline(small_batch):
     say(f"batch {ms_batch:.3f} ms, per-grid calls {ms_seq:.3f} ms, schedule {sch}")
 
 
+def test_batch_with_binned_grids_equals_tile_calls(small_batch):
+    grids, tables, devs, Q, offsets = small_batch
+    say("batch with binned grids")
+    batch = capi.Batch(devs)
+    dQ = capi.DeviceArray.from_host(Q)
+    counts = np.diff(offsets)
+    cut = int(np.median(counts[counts > 0]))
+    batch.set_schedule(128, 4, 1, 1, binned_min=cut)  # the larger half of the grids binned
+    assert batch.schedule()["binned_min"] == cut
+    for prec in (capi.FP32, capi.FP64):
+        vals, vecs = batch.outputs(offsets, prec)
+        assert batch.run(dQ, offsets, vals, vecs, precision=prec) == 0
+        v2, c2 = batch.outputs(offsets, prec)
+        assert batch.run(dQ, offsets, v2, c2, precision=prec, kernel=capi.KERNEL_TILE) == 0
+        for g in range(len(devs)):
+            np.testing.assert_array_equal(vals[g].to_host(), v2[g].to_host(), err_msg=f"grid {g}")
+            np.testing.assert_array_equal(vecs[g].to_host(), c2[g].to_host(), err_msg=f"grid {g}")
+    with pytest.raises(capi.BrilleError):
+        batch.set_schedule(100, 4, 1, 1)
+
+
 def test_batch_fp64_matches_oracle(small_batch):
     grids, tables, devs, Q, offsets = small_batch
     say("batch fp64 vs oracle")

@@ -112,8 +112,8 @@ def test_fill_validation():
         g.fill(np.zeros((V, 6, 2)), np.array([0, 2, 0]), np.zeros((V, 6, 1), complex), np.array([1, 0, 0]))
     with pytest.raises(RuntimeError, match="RotatesLike"):
         g.fill(np.zeros((V, 6, 1)), np.array([1, 0, 0, 7]), np.zeros((V, 6, 1), complex), np.array([1, 0, 0]))
-    with pytest.raises(RuntimeError, match="sort"):
-        g.fill(np.zeros((V, 6, 1)), np.array([1, 0, 0]), np.zeros((V, 6, 1), complex), np.array([1, 0, 0]), True)
+    # sort=True is accepted; the sorting itself runs on the device at first use
+    g.fill(np.zeros((V, 6, 1)), np.array([1, 0, 0]), np.zeros((V, 6, 1), complex), np.array([1, 0, 0]), True)
 
 
 def test_fill_keeps_references():
