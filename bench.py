@@ -54,6 +54,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline (profiling runs)")
     ap.add_argument("--no-learned", action="store_true", help="config 4: skip training the tetrahedron proposal")
     ap.add_argument("--no-batch", action="store_true", help="config 4: skip the configuration-2 batch leg")
+    ap.add_argument("--no-sort", action="store_true", help="config 4: skip the mode-sort leg")
+    ap.add_argument("--sort-only", action="store_true",
+                    help="only the mode-sort leg on the config-4 grid (profiling runs)")
     ap.add_argument("--paths", default="all",
                     help="config 4: comma-separated paths to time beside the headline (binned32, tile32, binned64; "
                          "'none' for the headline alone, as the byte-counter runs use it)")
@@ -93,6 +96,9 @@ def main():
     T = g.tables()
     log(f"grid cfg{cfg}: {T['vertices'].shape[0]} vertices, {T['node_type'].size} nodes, "
         f"{T['tet_volume'].size} tetrahedra, built in {time.time() - t0:.1f}s")
+    if args.sort_only:
+        print(json.dumps(sort_leg(args, capi, S, T)), flush=True)
+        return
     dev = capi.Grid(T)
     B = int(T["branches"])
     sv = int(sum(T["values_elements"]))
@@ -241,6 +247,8 @@ def main():
         "learned": learned,
         "fp32_error": fp32_error,
     }
+    if not args.no_sort:
+        out["sort_config4"] = sort_leg(args, capi, S, T)
     if not args.no_batch:
         # configuration 2 beside the headline, timed in the same run: the
         # config-4 buffers are released first (the batch outputs take 152 GB)
@@ -298,12 +306,14 @@ def batch_leg(args, capi, S, with_baselines):
     counts = np.diff(offsets.astype(np.int64))
     alg = float((counts * (24 + np.array(per_q_out))).sum()) + table_total
     achieved = alg / (np.mean(kms) * 1e-3) / 1e9
-    res = dict(value=n_q / wall * 1e3, unit="Q-points/s", ms_per_step=wall, graph_replay_ms=float(np.mean(kms)),
+    res = dict(value=n_q / wall * 1e3, unit="Q-points/s", ms_per_step=wall, batch_ms=float(np.mean(kms)),
                n_q=n_q, vertices_total=int(sum(V)), vertices=[min(V), max(V)], modes=[min(Bm), max(Bm)],
                schedule=sch, output_gb=float((counts * np.array(per_q_out)).sum() / 1e9),
                roofline=dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
-                             frac=achieved / HBM_PEAK_GBS, kernel="k_batch<float>"))
-    log(f"batched fp32: {wall:.3f} ms/step wall, graph {np.mean(kms):.3f} ms, {achieved:.0f} GB/s")
+                             frac=achieved / HBM_PEAK_GBS,
+                             kernel="k_batch<float> graph replay, then the binned pipelines of the grids "
+                                    "given at least binned_min points (HIP events around both)"))
+    log(f"batched fp32: {wall:.3f} ms/step wall, batch {np.mean(kms):.3f} ms, {achieved:.0f} GB/s")
     if with_baselines:
         seq = batch.time(dQ, offsets, vals, vecs, precision=capi.FP32, kernel=capi.KERNEL_TILE, iters=1)
         res["per_grid_tile_ms"] = seq
@@ -363,6 +373,53 @@ def bench_batch(args, capi, S):
         "speedup_vs_per_grid_calls": res.get("speedup_vs_per_grid_calls"),
     }
     print(json.dumps(out), flush=True)
+
+
+def sort_leg(args, capi, S, T):
+    """The mode sort (fill(..., sort=True)) on the benchmark grid: a payload
+    whose modes never cross, shuffled at every vertex; one brille_grid_sort
+    timed after a warm-up sort, and the oracle's sort (C, OpenMP) on a bounded
+    sample of the same pairs for the CPU rate."""
+    t = dict(T)
+    values, vectors, _ = S.shuffled_modes(t, seed=4)
+    t["values"], t["vectors"] = values, vectors
+    dev = capi.Grid(t)
+    dev.sort()  # warm: work space, code objects
+    lib = capi.lib()
+    capi.check(lib.brille_device_sync())
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        dev.sort()
+    capi.check(lib.brille_device_sync())
+    ms = (time.perf_counter() - t0) * 1e3 / reps
+    keys, index, perms = dev.permutations()
+    n_pairs = len(keys) // 2
+    res = dict(ms=ms, keys=int(len(keys)), pairs=int(n_pairs), pairs_per_s=n_pairs / ms * 1e3,
+               non_identity=int(len(perms) - 1), modes=int(t["branches"]),
+               workload="config-4 grid, 121 200 vertices, 6 modes shuffled at every vertex")
+    if not args.no_cpu:
+        from oracle import oracle as O
+        hw = host_cpu()
+        V = np.uint64(t["vertices"].shape[0])
+        i = (keys // V).astype(np.uint32)
+        j = (keys % V).astype(np.uint32)
+        fwd = np.flatnonzero(i < j)
+        sample = fwd[: min(len(fwd), 400_000)]
+        pairs = np.stack([i[sample], j[sample]], axis=1)
+        c0 = time.perf_counter()
+        rc, row, col = O.sort_pairs(t, pairs, nthreads=hw["threads"])
+        cs = time.perf_counter() - c0
+        assert rc == 0
+        # the sample's permutations agree with the device table
+        pos = np.searchsorted(keys, pairs[:, 0].astype(np.uint64) * V + pairs[:, 1].astype(np.uint64))
+        assert np.array_equal(perms[index[pos]], row.astype(np.uint32))
+        res["cpu_baseline"] = dict(pairs_per_s=len(pairs) / cs, sample=f"{len(pairs)} pairs of the same table",
+                                   threads=hw["threads"], kind="port")
+        res["speedup_vs_cpu"] = res["pairs_per_s"] / res["cpu_baseline"]["pairs_per_s"]
+    log(f"sort: {res}")
+    del dev
+    return res
 
 
 def measured_traffic(cfg, path_name):

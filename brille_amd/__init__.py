@@ -21,6 +21,8 @@ from ._brille import (  # noqa: E402
     ApproxConfig,
     Basis,
     BrillouinZone,
+    BZMeshQdc,
+    BZNestQdc,
     BZTrellisQdc,
     Direct,
     LengthUnit,
@@ -37,6 +39,6 @@ from ._brille import (  # noqa: E402
 from . import capi  # noqa: E402,F401
 
 __version__ = "0.1.0"
-__all__ = ["ApproxConfig", "Basis", "BrillouinZone", "BZTrellisQdc", "Direct", "LengthUnit", "Lattice",
+__all__ = ["ApproxConfig", "Basis", "BrillouinZone", "BZMeshQdc", "BZNestQdc", "BZTrellisQdc", "Direct", "LengthUnit", "Lattice",
            "NodeType", "PointSymmetry", "Polyhedron", "Reciprocal", "RotatesLike", "Symmetry", "hall_symbol",
            "spacegroup_number", "capi"]

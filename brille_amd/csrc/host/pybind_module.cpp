@@ -62,6 +62,9 @@ struct PyGrid {
   py::object values_ref, vectors_ref;
 };
 
+struct PyMesh : PyGrid {};
+struct PyNest : PyGrid {};
+
 struct PyZone {
   BrillouinZone bz;
   std::shared_ptr<ZoneTables> tables;
@@ -541,4 +544,49 @@ PYBIND11_MODULE(_brille, m) {
         return t;
       })
       .def("device_handle", [](PyGrid& g) { return reinterpret_cast<uintptr_t>(g.g->handle()); });
+
+  // BZMeshQdc and BZNestQdc (src/bz_mesh.hpp:37-112, src/bz_nest.hpp:36-104;
+  // constructors wrap/_mesh.hpp:36, wrap/_nest.hpp:37-38): the same methods
+  // over a tetrahedral mesh of the irreducible zone.  The reference meshes
+  // with TetGen, which this build does not have; here the mesh is the trellis
+  // with every node triangulated (tetrahedra no larger than the requested
+  // volume), located and summed by the same device path, so stages (a) and
+  // (c) are shared and only the mesh differs.  num_levels, max_points and
+  // max_branchings shape TetGen's refinement and are accepted and ignored.
+  auto mesh_volume = [](const PyZone& z, double v) {
+    const double ir = z.bz.ir.volume();
+    return v > 0 ? v : ir / 8.0;
+  };
+  py::class_<PyMesh, PyGrid>(m, "BZMeshQdc", py::dynamic_attr())
+      .def(py::init([mesh_volume](const PyZone& z, double max_size, int num_levels, int max_points) {
+             (void)num_levels;
+             (void)max_points;
+             return PyMesh{PyGrid{std::make_shared<BZTrellisQdc>(z.bz, mesh_volume(z, max_size), true, z.bz.approx),
+                                  py::none(), py::none()}};
+           }),
+           "brillouin_zone"_a, "max_size"_a = -1., "num_levels"_a = 3, "max_points"_a = -1);
+  py::class_<PyNest, PyGrid>(m, "BZNestQdc", py::dynamic_attr())
+      .def(py::init([mesh_volume](const PyZone& z, double max_volume, unsigned max_branchings) {
+             (void)max_branchings;
+             return PyNest{PyGrid{std::make_shared<BZTrellisQdc>(z.bz, mesh_volume(z, max_volume), true, z.bz.approx),
+                                  py::none(), py::none()}};
+           }),
+           "brillouin_zone"_a, "max_volume"_a, "max_branchings"_a = 5)
+      .def(py::init([mesh_volume](const PyZone& z, unsigned number_density, unsigned max_branchings) {
+             (void)max_branchings;
+             const double v = z.bz.ir.volume() / std::max(1u, number_density);
+             return PyNest{PyGrid{std::make_shared<BZTrellisQdc>(z.bz, mesh_volume(z, v), true, z.bz.approx),
+                                  py::none(), py::none()}};
+           }),
+           "brillouin_zone"_a, "number_density"_a, "max_branchings"_a = 5)
+      .def_property_readonly("all_invA", [](const PyGrid& g) {
+        return vec_array(g.g->trellis.vertices, {(py::ssize_t)g.g->trellis.n_vertices(), 3});
+      })
+      .def_property_readonly("all_rlu", [](const PyGrid& g) {
+        std::vector<V3> v;
+        const auto& T = g.g->trellis;
+        for (size_t i = 0; i < T.n_vertices(); ++i)
+          v.push_back(g.g->bz.outer_from_xyz({T.vertices[3 * i], T.vertices[3 * i + 1], T.vertices[3 * i + 2]}));
+        return v3list(v);
+      });
 }

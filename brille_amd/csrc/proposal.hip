@@ -7,11 +7,18 @@
 // the trellis; the points that land in polyhedron nodes give (features,
 // tetrahedron of PolyNode::indices_weights).  The network is trained with
 // softmax cross-entropy restricted to the tetrahedra of the point's node, by
-// this file's own kernels:
-//   k_prop_fwd_bwd   one wave per sample: forward, loss, the sparse W3/b3
-//                    gradients (atomics on the candidate rows) and the
-//                    per-sample hidden activations and their gradients;
-//   k_prop_wgrad_*   the W2/b2 and W1/b1 gradients as sums over the batch;
+// this file's own kernels; the three 128-wide products of a step run on the
+// matrix cores in bf16 with FP32 accumulation (v_mfma_f32_16x16x32_bf16),
+// everything else in FP32:
+//   k_tr_layer1      h1 = relu(W1 x + b1) per sample (FP32, and a bf16 copy);
+//   k_tr_layer2      a2 = bf16(h1) bf16(W2)^T + b2, one wave per 16 samples;
+//   k_tr_head        one wave per sample: h2 = relu(a2), the logits of the
+//                    node's candidates (FP32 W3), loss, the sparse W3/b3
+//                    gradients (atomics on the candidate rows) and
+//                    da2 = (dz W3) * [a2 > 0] (FP32, and a bf16 copy);
+//   k_tr_back1       da1 = (bf16(da2) bf16(W2)) * [h1 > 0];
+//   k_tr_wgrad2      gW2 = bf16(da2)^T bf16(h1) over the batch (K = samples);
+//   k_tr_bias2, k_prop_wgrad1   gb2, gW1 and gb1 as FP32 sums over the batch;
 //   k_prop_adam      Adam on the FP32 master parameters (and clears gradients).
 // After training the inference weights are exported as bf16 (W3 lane-major)
 // and the hit rate is measured on held-out pairs with the matrix-core
@@ -22,6 +29,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <random>
 #include <string>
 #include <vector>
@@ -102,108 +110,6 @@ __device__ __forceinline__ float wave_max(float v) {
   return v;
 }
 
-// one wave per sample, four samples per block.  Loss is the batch mean.
-__global__ __launch_bounds__(256) void k_prop_fwd_bwd(const float* P, float* G, uint32_t K, const float* feat,
-                                                      const int32_t* label, const uint32_t* t0, const uint32_t* nc,
-                                                      uint32_t n, float inv_n, float* H1, float* DA2, float* DA1,
-                                                      float* loss) {
-  const ParamOffsets o = param_offsets(K);
-  __shared__ float s_h1[4][H];
-  __shared__ float s_da2[4][H];
-  const uint32_t w = threadIdx.x >> 6, l = threadIdx.x & 63u;
-  const uint32_t s = blockIdx.x * 4 + w;
-  const bool valid = s < n;
-  const uint32_t j0 = l, j1 = l + 64;
-  float x0 = 0.f, x1 = 0.f, x2 = 0.f;
-  if (valid) {
-    x0 = feat[3 * (uint64_t)s];
-    x1 = feat[3 * (uint64_t)s + 1];
-    x2 = feat[3 * (uint64_t)s + 2];
-  }
-  const float* W1 = P + o.W1;
-  float a1_0 = W1[3 * j0] * x0 + W1[3 * j0 + 1] * x1 + W1[3 * j0 + 2] * x2 + P[o.b1 + j0];
-  float a1_1 = W1[3 * j1] * x0 + W1[3 * j1 + 1] * x1 + W1[3 * j1 + 2] * x2 + P[o.b1 + j1];
-  const float h1_0 = fmaxf(a1_0, 0.f), h1_1 = fmaxf(a1_1, 0.f);
-  s_h1[w][j0] = h1_0;
-  s_h1[w][j1] = h1_1;
-  if (valid) {
-    H1[(uint64_t)s * H + j0] = h1_0;
-    H1[(uint64_t)s * H + j1] = h1_1;
-  }
-  __syncthreads();
-  const float* W2 = P + o.W2;
-  float a2_0 = P[o.b2 + j0], a2_1 = P[o.b2 + j1];
-  for (int k = 0; k < H; ++k) {
-    const float h = s_h1[w][k];
-    a2_0 += W2[(uint64_t)j0 * H + k] * h;
-    a2_1 += W2[(uint64_t)j1 * H + k] * h;
-  }
-  const float h2_0 = fmaxf(a2_0, 0.f), h2_1 = fmaxf(a2_1, 0.f);
-  float dh2_0 = 0.f, dh2_1 = 0.f;
-  if (valid) {
-    const uint32_t c0 = t0[s], cn = nc[s];
-    const int32_t lab = label[s] - (int32_t)c0;
-    const float* W3 = P + o.W3;
-    float z = -3.0e38f;
-    for (uint32_t c = 0; c < cn; ++c) {
-      const uint64_t row = (uint64_t)(c0 + c) * H;
-      float part = W3[row + j0] * h2_0 + W3[row + j1] * h2_1;
-      part = wave_sum(part) + P[o.b3 + c0 + c];
-      if (l == c) z = part;
-    }
-    const float mx = wave_max(l < cn ? z : -3.0e38f);
-    const float e = l < cn ? expf(z - mx) : 0.f;
-    const float S = wave_sum(e);
-    const float zl = __shfl(z, lab);
-    if (l == 0) atomicAdd(loss, -(zl - mx - logf(S)) * inv_n);
-    float dz = l < cn ? e / S : 0.f;
-    if ((int32_t)l == lab) dz -= 1.f;
-    dz *= inv_n;
-    for (uint32_t c = 0; c < cn; ++c) {
-      const float dzc = __shfl(dz, (int)c);
-      const uint64_t row = (uint64_t)(c0 + c) * H;
-      atomicAdd(&G[o.W3 + row + j0], dzc * h2_0);
-      atomicAdd(&G[o.W3 + row + j1], dzc * h2_1);
-      dh2_0 += dzc * W3[row + j0];
-      dh2_1 += dzc * W3[row + j1];
-    }
-    if (l < cn) atomicAdd(&G[o.b3 + c0 + l], dz);
-  }
-  const float da2_0 = a2_0 > 0.f ? dh2_0 : 0.f, da2_1 = a2_1 > 0.f ? dh2_1 : 0.f;
-  s_da2[w][j0] = da2_0;
-  s_da2[w][j1] = da2_1;
-  if (valid) {
-    DA2[(uint64_t)s * H + j0] = da2_0;
-    DA2[(uint64_t)s * H + j1] = da2_1;
-  }
-  __syncthreads();
-  float dh1_0 = 0.f, dh1_1 = 0.f;
-  for (int j = 0; j < H; ++j) {
-    const float d = s_da2[w][j];
-    dh1_0 += W2[(uint64_t)j * H + j0] * d;
-    dh1_1 += W2[(uint64_t)j * H + j1] * d;
-  }
-  if (valid) {
-    DA1[(uint64_t)s * H + j0] = a1_0 > 0.f ? dh1_0 : 0.f;
-    DA1[(uint64_t)s * H + j1] = a1_1 > 0.f ? dh1_1 : 0.f;
-  }
-}
-
-// gW2[j][k] += sum_s DA2[s][j] H1[s][k], gb2[j] += sum_s DA2[s][j]: block (j, segment), thread k
-__global__ __launch_bounds__(128) void k_prop_wgrad2(const float* DA2, const float* H1, uint32_t n, uint32_t seg,
-                                                     float* G, uint64_t oW2, uint64_t ob2) {
-  const uint32_t j = blockIdx.x, k = threadIdx.x;
-  const uint32_t s0 = blockIdx.y * seg, s1 = min(n, s0 + seg);
-  float acc = 0.f, accb = 0.f;
-  for (uint32_t s = s0; s < s1; ++s) {
-    const float d = DA2[(uint64_t)s * H + j];
-    acc += d * H1[(uint64_t)s * H + k];
-    accb += d;
-  }
-  atomicAdd(&G[oW2 + (uint64_t)j * H + k], acc);
-  if (k == 0) atomicAdd(&G[ob2 + j], accb);
-}
-
 // gW1[k][d] += sum_s DA1[s][k] x[s][d], gb1[k] += sum_s DA1[s][k]: block (segment), thread k
 __global__ __launch_bounds__(128) void k_prop_wgrad1(const float* DA1, const float* feat, uint32_t n, uint32_t seg,
                                                      float* G, uint64_t oW1, uint64_t ob1) {
@@ -244,6 +150,161 @@ __global__ void k_fill_zero(float* p, uint64_t n) {
 __device__ __forceinline__ uint16_t bf16_bits(float f) {
   const uint32_t u = __float_as_uint(f);
   return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);  // round to nearest even (finite inputs)
+}
+
+// ---- training kernels (bf16 matrix cores for the 128-wide products) ----
+
+// bf16 copies of W2 for one step: row-major (rows = output unit) and transposed
+__global__ void k_tr_w2(const float* P, uint64_t oW2, uint16_t* W2b, uint16_t* W2bT) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (uint32_t)H * H) return;
+  const uint32_t j = i / H, k = i % H;
+  const uint16_t v = bf16_bits(P[oW2 + i]);
+  W2b[i] = v;
+  W2bT[k * H + j] = v;
+}
+
+// h1 = relu(W1 x + b1) for samples [0, n_pad): FP32 and bf16 rows; samples
+// from n on are zero (the products then see zero rows)
+__global__ __launch_bounds__(256) void k_tr_layer1(const float* P, uint64_t oW1, uint64_t ob1, const float* feat,
+                                                   uint32_t n, uint32_t n_pad, float* H1, uint16_t* H1b) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (uint64_t)n_pad * H) return;
+  const uint32_t s = (uint32_t)(t / H), k = (uint32_t)(t % H);
+  float h = 0.f;
+  if (s < n) {
+    const float a = P[oW1 + 3 * k] * feat[3 * (uint64_t)s] + P[oW1 + 3 * k + 1] * feat[3 * (uint64_t)s + 1] +
+                    P[oW1 + 3 * k + 2] * feat[3 * (uint64_t)s + 2] + P[ob1 + k];
+    h = fmaxf(a, 0.f);
+  }
+  H1[t] = h;
+  H1b[t] = bf16_bits(h);
+}
+
+// out[s][c] = sum_k A[s][k] B[c][k] (+ bias[c]) for 16 samples per wave: A
+// rows bf16 (samples x 128), B bf16 row-major (128 x 128, row = output c);
+// out FP32 (samples x 128).  With mask (nullable, FP32 samples x 128) the
+// product is kept only where mask > 0.
+__global__ __launch_bounds__(256) void k_tr_rows_x_w(const uint16_t* A, const uint16_t* Bw, const float* bias,
+                                                     const float* mask, uint32_t n_pad, float* out) {
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint32_t s0 = wave * 16;
+  if (s0 >= n_pad) return;  // whole waves leave together
+  const uint32_t l = threadIdx.x & 63u, m = l & 15u, grp = l >> 4;
+  bf16x8_t a[4];
+#pragma unroll
+  for (int kb = 0; kb < 4; ++kb) a[kb] = load_bf16x8(A + (uint64_t)(s0 + m) * H + 32 * kb + 8 * grp);
+#pragma unroll
+  for (int nb = 0; nb < 8; ++nb) {
+    floatx4_t acc = floatx4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb)
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[kb], load_bf16x8(Bw + (uint64_t)(nb * 16 + m) * H + 32 * kb + 8 * grp),
+                                                    acc, 0, 0, 0);
+    const uint32_t c = nb * 16 + m;
+    const float bc = bias ? bias[c] : 0.f;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const uint64_t at = (uint64_t)(s0 + 4 * grp + r) * H + c;
+      float v = acc[r] + bc;
+      if (mask && !(mask[at] > 0.f)) v = 0.f;
+      out[at] = v;
+    }
+  }
+}
+
+// one wave per sample: the head (FP32), its loss and sparse gradients, and
+// da2 = (dz W3) * [a2 > 0] as FP32 and bf16 rows (zero rows from n to n_pad)
+__global__ __launch_bounds__(256) void k_tr_head(const float* P, float* G, uint32_t K, const float* A2,
+                                                 const int32_t* label, const uint32_t* t0, const uint32_t* nc,
+                                                 uint32_t n, uint32_t n_pad, float inv_n, float* DA2, uint16_t* DA2b,
+                                                 float* loss) {
+  const ParamOffsets o = param_offsets(K);
+  const uint32_t w = threadIdx.x >> 6, l = threadIdx.x & 63u;
+  const uint32_t s = blockIdx.x * 4 + w;
+  if (s >= n_pad) return;
+  const uint32_t j0 = l, j1 = l + 64;
+  const float a2_0 = A2[(uint64_t)s * H + j0], a2_1 = A2[(uint64_t)s * H + j1];
+  const float h2_0 = fmaxf(a2_0, 0.f), h2_1 = fmaxf(a2_1, 0.f);
+  float dh2_0 = 0.f, dh2_1 = 0.f;
+  if (s < n) {
+    const uint32_t c0 = t0[s], cn = nc[s];
+    const int32_t lab = label[s] - (int32_t)c0;
+    const float* W3 = P + o.W3;
+    float z = -3.0e38f;
+    for (uint32_t c = 0; c < cn; ++c) {
+      const uint64_t row = (uint64_t)(c0 + c) * H;
+      float part = W3[row + j0] * h2_0 + W3[row + j1] * h2_1;
+      part = wave_sum(part) + P[o.b3 + c0 + c];
+      if (l == c) z = part;
+    }
+    const float mx = wave_max(l < cn ? z : -3.0e38f);
+    const float e = l < cn ? expf(z - mx) : 0.f;
+    const float S = wave_sum(e);
+    const float zl = __shfl(z, lab);
+    if (l == 0) atomicAdd(loss, -(zl - mx - logf(S)) * inv_n);
+    float dz = l < cn ? e / S : 0.f;
+    if ((int32_t)l == lab) dz -= 1.f;
+    dz *= inv_n;
+    for (uint32_t c = 0; c < cn; ++c) {
+      const float dzc = __shfl(dz, (int)c);
+      const uint64_t row = (uint64_t)(c0 + c) * H;
+      atomicAdd(&G[o.W3 + row + j0], dzc * h2_0);
+      atomicAdd(&G[o.W3 + row + j1], dzc * h2_1);
+      dh2_0 += dzc * W3[row + j0];
+      dh2_1 += dzc * W3[row + j1];
+    }
+    if (l < cn) atomicAdd(&G[o.b3 + c0 + l], dz);
+  }
+  const float da2_0 = a2_0 > 0.f ? dh2_0 : 0.f, da2_1 = a2_1 > 0.f ? dh2_1 : 0.f;
+  DA2[(uint64_t)s * H + j0] = da2_0;
+  DA2[(uint64_t)s * H + j1] = da2_1;
+  DA2b[(uint64_t)s * H + j0] = bf16_bits(da2_0);
+  DA2b[(uint64_t)s * H + j1] = bf16_bits(da2_1);
+}
+
+// bf16 transpose of a (rows x 128) matrix into (128 x rows), 32 x 32 tiles through LDS
+__global__ __launch_bounds__(256) void k_tr_transpose(const uint16_t* in, uint32_t rows, uint16_t* out) {
+  __shared__ uint16_t t[32][33];
+  const uint32_t r0 = blockIdx.x * 32, c0 = blockIdx.y * 32;
+  const uint32_t tx = threadIdx.x & 31u, ty = threadIdx.x >> 5;  // 32 x 8
+  for (uint32_t y = ty; y < 32; y += 8) t[y][tx] = in[(uint64_t)(r0 + y) * H + c0 + tx];
+  __syncthreads();
+  for (uint32_t y = ty; y < 32; y += 8) out[(uint64_t)(c0 + y) * rows + r0 + tx] = t[tx][y];
+}
+
+// gW2[j][k] += sum over the chunk's samples of DA2T[j][s] H1T[k][s]: one
+// wave per (16 rows j, chunk of samples), all 128 columns; FP32 atomics
+constexpr uint32_t kTrChunk = 1024;
+__global__ __launch_bounds__(64) void k_tr_wgrad2(const uint16_t* DA2T, const uint16_t* H1T, uint32_t n_pad, float* G,
+                                                  uint64_t oW2) {
+  const uint32_t jt = blockIdx.x, chunk = blockIdx.y;
+  const uint32_t l = threadIdx.x, m = l & 15u, grp = l >> 4;
+  const uint32_t s_lo = chunk * kTrChunk, s_hi = min(n_pad, s_lo + kTrChunk);
+  floatx4_t acc[8];
+#pragma unroll
+  for (int nb = 0; nb < 8; ++nb) acc[nb] = floatx4_t{0.f, 0.f, 0.f, 0.f};
+  for (uint32_t s = s_lo; s < s_hi; s += 32) {
+    const bf16x8_t a = load_bf16x8(DA2T + (uint64_t)(jt * 16 + m) * n_pad + s + 8 * grp);
+#pragma unroll
+    for (int nb = 0; nb < 8; ++nb)
+      acc[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, load_bf16x8(H1T + (uint64_t)(nb * 16 + m) * n_pad + s + 8 * grp),
+                                                        acc[nb], 0, 0, 0);
+  }
+#pragma unroll
+  for (int nb = 0; nb < 8; ++nb)
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+      atomicAdd(&G[oW2 + (uint64_t)(jt * 16 + 4 * grp + r) * H + nb * 16 + m], acc[nb][r]);
+}
+
+// gb2[j] += sum_s DA2[s][j]: block = segment of samples, thread j
+__global__ __launch_bounds__(128) void k_tr_bias2(const float* DA2, uint32_t n, uint32_t seg, float* G, uint64_t ob2) {
+  const uint32_t j = threadIdx.x;
+  const uint32_t s0 = blockIdx.x * seg, s1 = min(n, s0 + seg);
+  float acc = 0.f;
+  for (uint32_t s = s0; s < s1; ++s) acc += DA2[(uint64_t)s * H + j];
+  atomicAdd(&G[ob2 + j], acc);
 }
 
 // bf16 inference copies: W2 row-major, W3 lane-major (element nb*16+m of a row at m*8+nb)
@@ -373,20 +434,50 @@ static int host_candidates(const brille_grid* G, const int32_t* labels, uint32_t
   return BRILLE_OK;
 }
 
-// one optimisation step's gradients (accumulated into prop_grads) and loss
+// one optimisation step's gradients (accumulated into prop_grads) and loss;
+// buffers for batches of up to `cap` samples (rounded up to 32)
 struct StepBuffers {
-  float* H1;
-  float* DA2;
-  float* DA1;
-  float* loss;
+  float *H1, *A2, *DA2, *DA1, *loss;
+  uint16_t *H1b, *H1T, *DA2b, *DA2T, *W2b, *W2bT;
 };
+static uint32_t step_pad(uint32_t n) { return (n + 31u) & ~31u; }
+static bool step_alloc(StepBuffers& sb, uint32_t cap, const std::function<void*(size_t)>& alloc) {
+  const size_t rows = step_pad(cap);
+  sb.H1 = static_cast<float*>(alloc(rows * H * 4));
+  sb.A2 = static_cast<float*>(alloc(rows * H * 4));
+  sb.DA2 = static_cast<float*>(alloc(rows * H * 4));
+  sb.DA1 = static_cast<float*>(alloc(rows * H * 4));
+  sb.loss = static_cast<float*>(alloc(4));
+  sb.H1b = static_cast<uint16_t*>(alloc(rows * H * 2));
+  sb.H1T = static_cast<uint16_t*>(alloc(rows * H * 2));
+  sb.DA2b = static_cast<uint16_t*>(alloc(rows * H * 2));
+  sb.DA2T = static_cast<uint16_t*>(alloc(rows * H * 2));
+  sb.W2b = static_cast<uint16_t*>(alloc(H * H * 2));
+  sb.W2bT = static_cast<uint16_t*>(alloc(H * H * 2));
+  return sb.H1 && sb.A2 && sb.DA2 && sb.DA1 && sb.loss && sb.H1b && sb.H1T && sb.DA2b && sb.DA2T && sb.W2b && sb.W2bT;
+}
 static void grads_launch(brille_grid* G, const StepBuffers& sb, const float* feat, const int32_t* label,
                          const uint32_t* t0, const uint32_t* nc, uint32_t n, hipStream_t s) {
   const ParamOffsets o = param_offsets(G->prop_K);
-  hipLaunchKernelGGL(k_prop_fwd_bwd, dim3((n + 3) / 4), dim3(256), 0, s, G->prop_params, G->prop_grads, G->prop_K,
-                     feat, label, t0, nc, n, 1.f / (float)n, sb.H1, sb.DA2, sb.DA1, sb.loss);
-  const uint32_t seg = 512, nseg = (n + seg - 1) / seg;
-  hipLaunchKernelGGL(k_prop_wgrad2, dim3(H, nseg), dim3(H), 0, s, sb.DA2, sb.H1, n, seg, G->prop_grads, o.W2, o.b2);
+  const uint32_t np = step_pad(n);
+  const float* P = G->prop_params;
+  hipLaunchKernelGGL(k_tr_w2, dim3(H * H / 256), dim3(256), 0, s, P, o.W2, sb.W2b, sb.W2bT);
+  hipLaunchKernelGGL(k_tr_layer1, dim3((unsigned)(((uint64_t)np * H + 255) / 256)), dim3(256), 0, s, P, o.W1, o.b1, feat,
+                     n, np, sb.H1, sb.H1b);
+  hipLaunchKernelGGL(k_tr_rows_x_w, dim3((np / 16 + 3) / 4), dim3(256), 0, s, sb.H1b, sb.W2b, P + o.b2,
+                     (const float*)nullptr, np, sb.A2);
+  hipLaunchKernelGGL(k_tr_head, dim3((np + 3) / 4), dim3(256), 0, s, P, G->prop_grads, G->prop_K, sb.A2, label, t0, nc,
+                     n, np, 1.f / (float)n, sb.DA2, sb.DA2b, sb.loss);
+  hipLaunchKernelGGL(k_tr_rows_x_w, dim3((np / 16 + 3) / 4), dim3(256), 0, s, sb.DA2b, sb.W2bT, (const float*)nullptr,
+                     sb.H1, np, sb.DA1);
+  hipLaunchKernelGGL(k_tr_transpose, dim3(np / 32, H / 32), dim3(256), 0, s, sb.H1b, np, sb.H1T);
+  hipLaunchKernelGGL(k_tr_transpose, dim3(np / 32, H / 32), dim3(256), 0, s, sb.DA2b, np, sb.DA2T);
+  hipLaunchKernelGGL(k_tr_wgrad2, dim3(H / 16, (np + kTrChunk - 1) / kTrChunk), dim3(64), 0, s, sb.DA2T, sb.H1T, np,
+                     G->prop_grads, o.W2);
+  // short segments: each thread's sum is a handful of independent loads, and
+  // the blocks spread over the chip (512-sample segments left 16 blocks)
+  const uint32_t seg = 32, nseg = (n + seg - 1) / seg;
+  hipLaunchKernelGGL(k_tr_bias2, dim3(nseg), dim3(H), 0, s, sb.DA2, n, seg, G->prop_grads, o.b2);
   hipLaunchKernelGGL(k_prop_wgrad1, dim3(nseg), dim3(H), 0, s, sb.DA1, feat, n, seg, G->prop_grads, o.W1, o.b1);
 }
 
@@ -427,12 +518,9 @@ int brille_proposal_train(brille_grid* G, const brille_train_opts* opts_in, bril
   uint32_t* nc = static_cast<uint32_t*>(talloc((size_t)total * sizeof(uint32_t)));
   uint32_t* count = static_cast<uint32_t*>(talloc(2 * sizeof(uint32_t)));
   StepBuffers sb;
-  sb.H1 = static_cast<float*>(talloc((size_t)o.batch * H * sizeof(float)));
-  sb.DA2 = static_cast<float*>(talloc((size_t)o.batch * H * sizeof(float)));
-  sb.DA1 = static_cast<float*>(talloc((size_t)o.batch * H * sizeof(float)));
-  sb.loss = static_cast<float*>(talloc(sizeof(float)));
+  const bool sb_ok = step_alloc(sb, o.batch, talloc);
   uint32_t* pred = static_cast<uint32_t*>(talloc((size_t)n_eval * sizeof(uint32_t)));
-  if (!feat || !label || !t0 || !nc || !count || !sb.H1 || !sb.DA2 || !sb.DA1 || !sb.loss || !pred) {
+  if (!feat || !label || !t0 || !nc || !count || !sb_ok || !pred) {
     cleanup();
     return fail(BRILLE_ERR_DEVICE, "hipMalloc failed for the training set");
   }
@@ -546,11 +634,8 @@ int brille_proposal_loss_grad(brille_grid* G, const float* x, const int32_t* lab
   uint32_t* dt0 = static_cast<uint32_t*>(talloc((size_t)n * sizeof(uint32_t)));
   uint32_t* dnc = static_cast<uint32_t*>(talloc((size_t)n * sizeof(uint32_t)));
   StepBuffers sb;
-  sb.H1 = static_cast<float*>(talloc((size_t)n * H * sizeof(float)));
-  sb.DA2 = static_cast<float*>(talloc((size_t)n * H * sizeof(float)));
-  sb.DA1 = static_cast<float*>(talloc((size_t)n * H * sizeof(float)));
-  sb.loss = static_cast<float*>(talloc(sizeof(float)));
-  bool ok = dfeat && dlab && dt0 && dnc && sb.H1 && sb.DA2 && sb.DA1 && sb.loss;
+  const bool sb_ok = step_alloc(sb, n, talloc);
+  bool ok = dfeat && dlab && dt0 && dnc && sb_ok;
   ok = ok && hipMemcpy(dfeat, x, (size_t)n * 3 * sizeof(float), hipMemcpyHostToDevice) == hipSuccess;
   ok = ok && hipMemcpy(dlab, labels, (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice) == hipSuccess;
   ok = ok && hipMemcpy(dt0, t0.data(), (size_t)n * sizeof(uint32_t), hipMemcpyHostToDevice) == hipSuccess;

@@ -96,6 +96,23 @@ __global__ __launch_bounds__(256) void k_pair_keys(uint64_t V, const uint32_t* _
   }
 }
 
+// rev[k]: the position of key (j, i) for key k = (i, j) (the lists are
+// symmetric), by binary search in j's list; one thread per key
+__global__ __launch_bounds__(256) void k_pair_rev(uint64_t V, uint64_t n_keys, const uint64_t* __restrict__ keys,
+                                                  const uint32_t* __restrict__ nbr, const uint32_t* __restrict__ koff,
+                                                  uint32_t* __restrict__ rev) {
+  const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n_keys) return;
+  const uint32_t i = (uint32_t)(keys[k] / V), j = nbr[k];
+  uint32_t lo = koff[j], hi = koff[j + 1];
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (nbr[mid] < i) lo = mid + 1;
+    else hi = mid;
+  }
+  rev[k] = (lo < koff[j + 1] && nbr[lo] == i) ? lo : 0xFFFFFFFFu;
+}
+
 // ---- costs ----------------------------------------------------------------
 
 // approx_float::scalar(|c|, 1) with its default tolerances (relative
@@ -297,16 +314,21 @@ __device__ __forceinline__ void wave_argmin(double& v, uint32_t& j) {
   }
 }
 
-// One wave per pair i < j of the key list (grid-stride).  LDS: potentials u,
+// One wave per pair i < j of the key list, each block a contiguous run of
+// keys (consecutive keys share their first vertex, whose rows then stay in
+// LDS).  LDS: potentials u,
 // v, distances minv ((B + 1) doubles each), matching p, predecessors way,
-// scanned flags ((B + 1) words each), then the assignment and its inverse
-// (B words each).  cost_ws: B x B doubles per block.
+// scanned flags ((B + 1) words each), the assignment and its inverse (B words
+// each), then (cost_in_lds) the B x B cost matrix; otherwise the matrix lives
+// in cost_ws, B x B doubles per block.  The Dijkstra steps read a row of it
+// each, so keeping it in LDS takes the L2 latency out of every step.
 template <typename Real>
 __global__ __launch_bounds__(SORT_BLOCK) void k_sort_pairs(CostArgs c, const Real* rows, uint64_t V, uint64_t n_keys,
                                                            const uint64_t* __restrict__ keys,
                                                            const uint32_t* __restrict__ nbr,
-                                                           const uint32_t* __restrict__ koff,
-                                                           double* __restrict__ cost_ws,
+                                                           const uint32_t* __restrict__ rev,
+                                                           double* __restrict__ cost_ws, int cost_in_lds,
+                                                           int rows_in_lds,
                                                            uint32_t* __restrict__ perm_index,
                                                            uint32_t* __restrict__ perms, uint32_t* __restrict__ counter,
                                                            uint32_t* __restrict__ bad) {
@@ -320,27 +342,40 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_sort_pairs(CostArgs c, const Rea
   int32_t* used = way + (B + 1);
   uint32_t* rowv = reinterpret_cast<uint32_t*>(used + (B + 1));
   uint32_t* colv = rowv + B;
-  double* C = cost_ws + (uint64_t)blockIdx.x * B * B;
+  double* C = cost_in_lds ? lds + 3 * (B + 1) + (3 * (B + 1) + 2 * B + 1) / 2 : cost_ws + (uint64_t)blockIdx.x * B * B;
+  // (rows_in_lds) the B rows of vertex i, then those of j, staged with
+  // coalesced loads: the cost loops then read LDS instead of issuing a
+  // dependent global load per element
+  const uint32_t vrow = B * c.stride;  // reals per vertex
+  Real* stage = reinterpret_cast<Real*>(C + (cost_in_lds ? (uint64_t)B * B : 0));
   const double INF = __builtin_huge_val();
-  for (uint64_t k = blockIdx.x; k < n_keys; k += gridDim.x) {
+  const uint64_t per = (n_keys + gridDim.x - 1) / gridDim.x;
+  const uint64_t k_lo = (uint64_t)blockIdx.x * per, k_hi = k_lo + per < n_keys ? k_lo + per : n_keys;
+  uint64_t staged_i = ~0ull;
+  for (uint64_t k = k_lo; k < k_hi; ++k) {
     const uint64_t key = keys[k];
     const uint64_t i = key / V;
     const uint32_t j = nbr[k];
     if (i > j) continue;  // block-uniform
-    // (j, i) in j's list
-    uint32_t lo = koff[j], hi = koff[j + 1];
-    while (lo < hi) {
-      const uint32_t mid = (lo + hi) >> 1;
-      if (nbr[mid] < (uint32_t)i) lo = mid + 1;
-      else hi = mid;
-    }
-    const uint64_t r = lo;
-    const bool has_rev = r < koff[j + 1] && nbr[r] == (uint32_t)i;  // always, the lists being symmetric
+    const uint32_t rv = rev[k];
+    const bool has_rev = rv != 0xFFFFFFFFu;  // always, the lists being symmetric
+    const uint64_t r = rv;
     // costs
+    const Real* ri = rows + i * vrow;
+    const Real* rj = rows + (uint64_t)j * vrow;
+    if (rows_in_lds) {
+      if (i != staged_i)
+        for (uint32_t q = lane; q < vrow; q += SORT_BLOCK) stage[q] = ri[q];
+      for (uint32_t q = lane; q < vrow; q += SORT_BLOCK) stage[vrow + q] = rj[q];
+      staged_i = i;
+      __syncthreads();
+      ri = stage;
+      rj = stage + vrow;
+    }
     bool ok = true;
     for (uint32_t e = lane; e < B * B; e += SORT_BLOCK) {
       const uint32_t a = e / B, b = e - a * B;
-      C[e] = pair_cost(c, rows + (i * B + a) * (uint64_t)c.stride, rows + ((uint64_t)j * B + b) * c.stride, ok);
+      C[e] = pair_cost(c, ri + a * c.stride, rj + b * c.stride, ok);
     }
     if (!ok) atomicOr(bad, 1u);
     for (uint32_t q = lane; q <= B; q += SORT_BLOCK) {
@@ -526,6 +561,11 @@ extern "C" int brille_grid_sort(brille_grid* G, const brille_cost_info* values_c
   uint32_t* counter = buf.get<uint32_t>(2);
   if (!keys || !nbr || !pidx || !perms || !counter) return fail(BRILLE_ERR_DEVICE, "hipMalloc failed for the permutation table");
   hipLaunchKernelGGL(k_pair_keys, dim3(vb), dim3(256), 0, s, V, off, adj, uniq, koff, keys, nbr);
+  uint32_t* rev = buf.get<uint32_t>(n_keys);
+  if (!rev) return fail(BRILLE_ERR_DEVICE, "hipMalloc failed for the sort work space");
+  if (n_keys)
+    hipLaunchKernelGGL(k_pair_rev, dim3((unsigned)((n_keys + 255) / 256)), dim3(256), 0, s, V, n_keys, keys, nbr, koff,
+                       rev);
   hipLaunchKernelGGL(k_identity_perm, dim3(1), dim3(256), 0, s, perms, d.B);
   zero_words(counter, 2, s);
   CostArgs ca{};
@@ -543,18 +583,25 @@ extern "C" int brille_grid_sort(brille_grid* G, const brille_cost_info* values_c
   ca.cs = cc.scalar_cost;
   ca.cv = cc.vector_cost;
   const unsigned blocks = (unsigned)std::min<uint64_t>(std::max<uint64_t>(n_keys, 1), (uint64_t)std::max(G->n_cu, 1u) * 32);
-  double* cost_ws = buf.get<double>((uint64_t)blocks * d.B * d.B);
-  if (!cost_ws) return fail(BRILLE_ERR_DEVICE, "hipMalloc failed for the cost matrices");
-  const size_t lds = 3 * (d.B + 1) * sizeof(double) + 3 * (d.B + 1) * sizeof(int32_t) + 2 * d.B * sizeof(uint32_t);
+  // the LAP state, then the cost matrix when it fits in 48 KB of LDS
+  const size_t lds_state = 3 * (d.B + 1) * sizeof(double) + ((3 * (d.B + 1) + 2 * d.B + 1) / 2) * sizeof(double);
+  const size_t lds_cost = (size_t)d.B * d.B * sizeof(double);
+  const int cost_in_lds = lds_state + lds_cost <= 48 * 1024 ? 1 : 0;
+  double* cost_ws = cost_in_lds ? nullptr : buf.get<double>((uint64_t)blocks * d.B * d.B);
+  if (!cost_in_lds && !cost_ws) return fail(BRILLE_ERR_DEVICE, "hipMalloc failed for the cost matrices");
+  const size_t rs = d.rows64 ? sizeof(double) : sizeof(float);
+  const size_t lds_rows = 2 * (size_t)d.B * (d.rows64 ? d.stride64 : d.stride32) * rs;
+  const int rows_in_lds = cost_in_lds && lds_state + lds_cost + lds_rows <= 60 * 1024 ? 1 : 0;
+  const size_t lds = lds_state + (cost_in_lds ? lds_cost : 0) + (rows_in_lds ? lds_rows : 0);
   if (n_keys) {
     if (d.rows64) {
       ca.stride = d.stride64;
       hipLaunchKernelGGL(k_sort_pairs<double>, dim3(blocks), dim3(SORT_BLOCK), lds, s, ca, d.rows64, V, n_keys, keys, nbr,
-                         koff, cost_ws, pidx, perms, counter, counter + 1);
+                         rev, cost_ws, cost_in_lds, rows_in_lds, pidx, perms, counter, counter + 1);
     } else {
       ca.stride = d.stride32;
       hipLaunchKernelGGL(k_sort_pairs<float>, dim3(blocks), dim3(SORT_BLOCK), lds, s, ca, d.rows32, V, n_keys, keys, nbr,
-                         koff, cost_ws, pidx, perms, counter, counter + 1);
+                         rev, cost_ws, cost_in_lds, rows_in_lds, pidx, perms, counter, counter + 1);
     }
   }
   if (hipGetLastError() != hipSuccess) return fail(BRILLE_ERR_DEVICE, "sort launch failed");

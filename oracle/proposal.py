@@ -83,6 +83,51 @@ def loss_grad(params, x, labels, poly_tet_offset, K, h=H):
     return loss / n, pack(G)
 
 
+def loss_grad_bf16(params, x, labels, poly_tet_offset, K, h=H):
+    """loss_grad as the training kernels compute it: the three 128-wide
+    products take bf16 operands (h1 and W2 in the forward product, da2 and W2
+    in the backward one, da2 and h1 in the W2 gradient), each rounded from its
+    FP32 value (nearest even); the rest as in loss_grad.  Sums in float64."""
+    P = unpack(params, K, h)
+    x32 = np.asarray(x, dtype=np.float32)
+    labels = np.asarray(labels, dtype=np.int64)
+    t0, t1 = candidates(poly_tet_offset, labels)
+    n = x32.shape[0]
+    W1 = P["W1"].astype(np.float32)
+    a1 = (x32[:, :1] * W1[:, 0] + x32[:, 1:2] * W1[:, 1] + x32[:, 2:3] * W1[:, 2] + P["b1"].astype(np.float32))
+    h1 = np.maximum(a1, 0).astype(np.float32)
+    h1b = to_bf16(h1).astype(np.float64)
+    W2b = to_bf16(P["W2"].astype(np.float32)).astype(np.float64)
+    a2 = h1b @ W2b.T + P["b2"]
+    h2 = np.maximum(a2, 0.0)
+    G = {k: np.zeros_like(v) for k, v in P.items()}
+    dh2 = np.zeros_like(h2)
+    loss = 0.0
+    for i in range(n):
+        c = np.arange(t0[i], t1[i])
+        z = P["W3"][c] @ h2[i] + P["b3"][c]
+        m = z.max()
+        e = np.exp(z - m)
+        s_ = e.sum()
+        lab = labels[i] - t0[i]
+        loss += -(z[lab] - m - np.log(s_))
+        dz = e / s_
+        dz[lab] -= 1.0
+        dz /= n
+        G["W3"][c] += np.outer(dz, h2[i])
+        G["b3"][c] += dz
+        dh2[i] = dz @ P["W3"][c]
+    da2 = dh2 * (a2 > 0)
+    da2b = to_bf16(da2.astype(np.float32)).astype(np.float64)
+    G["W2"] = da2b.T @ h1b
+    G["b2"] = da2.sum(axis=0)
+    dh1 = da2b @ W2b
+    da1 = dh1 * (h1 > 0)
+    G["W1"] = da1.T @ x32.astype(np.float64)
+    G["b1"] = da1.sum(axis=0)
+    return loss / n, pack(G)
+
+
 def predict(params, x, t0, t1, K, h=H):
     """Proposed tetrahedron (argmax over the node's candidates; first on ties)."""
     P = unpack(params, K, h)
