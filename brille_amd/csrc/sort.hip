@@ -314,26 +314,129 @@ __device__ __forceinline__ void wave_argmin(double& v, uint32_t& j) {
   }
 }
 
-// One wave per pair i < j of the key list, each block a contiguous run of
-// keys (consecutive keys share their first vertex, whose rows then stay in
-// LDS).  LDS: potentials u,
-// v, distances minv ((B + 1) doubles each), matching p, predecessors way,
-// scanned flags ((B + 1) words each), the assignment and its inverse (B words
-// each), then (cost_in_lds) the B x B cost matrix; otherwise the matrix lives
-// in cost_ws, B x B doubles per block.  The Dijkstra steps read a row of it
-// each, so keeping it in LDS takes the L2 latency out of every step.
+// forward pairs (i < j) per vertex: count, then (after a scan) the pair's
+// vertices and key index, in key order
+__global__ __launch_bounds__(256) void k_pair_fwd(uint64_t V, const uint32_t* __restrict__ koff,
+                                                  const uint32_t* __restrict__ nbr, const uint32_t* __restrict__ poff,
+                                                  uint32_t* __restrict__ cnt, uint32_t* __restrict__ pi,
+                                                  uint32_t* __restrict__ pj, uint32_t* __restrict__ pk) {
+  const uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= V) return;
+  uint32_t m = 0;
+  for (uint32_t k = koff[v]; k < koff[v + 1]; ++k) {
+    const uint32_t j = nbr[k];
+    if (j <= (uint32_t)v) continue;
+    if (poff) {
+      pi[poff[v] + m] = (uint32_t)v;
+      pj[poff[v] + m] = j;
+      pk[poff[v] + m] = k;
+    }
+    ++m;
+  }
+  if (!poff) cnt[v] = m;
+}
+
+// The cost matrices of pairs [p0, p0 + np): one lane per entry (pair, a, b),
+// so every lane of every wave computes (B x B entries per pair).
 template <typename Real>
-__global__ __launch_bounds__(SORT_BLOCK) void k_sort_pairs(CostArgs c, const Real* rows, uint64_t V, uint64_t n_keys,
-                                                           const uint64_t* __restrict__ keys,
-                                                           const uint32_t* __restrict__ nbr,
-                                                           const uint32_t* __restrict__ rev,
-                                                           double* __restrict__ cost_ws, int cost_in_lds,
-                                                           int rows_in_lds,
-                                                           uint32_t* __restrict__ perm_index,
-                                                           uint32_t* __restrict__ perms, uint32_t* __restrict__ counter,
-                                                           uint32_t* __restrict__ bad) {
+__global__ __launch_bounds__(256) void k_pair_costs(CostArgs c, const Real* rows, const uint32_t* __restrict__ pi,
+                                                    const uint32_t* __restrict__ pj, uint64_t p0, uint64_t np,
+                                                    double* __restrict__ C, uint32_t* __restrict__ bad) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t B = c.B, BB = B * B;
+  if (t >= np * BB) return;
+  const uint64_t p = t / BB;
+  const uint32_t e = (uint32_t)(t - p * BB);
+  const uint32_t a = e / B, b = e - a * B;
+  const uint64_t i = pi[p0 + p], j = pj[p0 + p];
+  bool ok = true;
+  C[t] = pair_cost(c, rows + (i * B + a) * c.stride, rows + (j * B + b) * c.stride, ok);
+  if (!ok) atomicOr(bad, 1u);
+}
+
+// The easy pairs first, one thread each: when the first minimum of every row
+// of the cost matrix lies in a different column, that column-per-row
+// assignment is the optimum and is exactly what the augmenting-path solver
+// below returns (each row's search then ends at its first minimum, a free
+// column).  Such pairs are written here; the others are flagged (hard[q] = 1)
+// for the full solver.  Matrices of more than 64 columns all go to the solver.
+__global__ __launch_bounds__(256) void k_pair_fast(const double* __restrict__ Cg, uint32_t B, uint64_t p0, uint64_t np,
+                                                   const uint32_t* __restrict__ pk, const uint32_t* __restrict__ rev,
+                                                   uint32_t* __restrict__ perm_index, uint32_t* __restrict__ perms,
+                                                   uint32_t* __restrict__ hard) {
+  const uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= np) return;
+  if (B > 64) {
+    hard[q] = 1;
+    return;
+  }
+  const double* C = Cg + q * B * B;
+  unsigned long long taken = 0;
+  bool moved = false, ok = true;
+  for (uint32_t a = 0; a < B && ok; ++a) {
+    double best = __builtin_huge_val();
+    uint32_t bc = 0xFFFFFFFFu;
+    for (uint32_t b = 0; b < B; ++b) {
+      const double c = C[(uint64_t)a * B + b];
+      if (c < best) {
+        best = c;
+        bc = b;
+      }
+    }
+    if (bc == 0xFFFFFFFFu || (taken >> bc) & 1ull) ok = false;
+    else taken |= 1ull << bc;
+    moved = moved || bc != a;
+  }
+  const uint32_t k = pk[p0 + q], rv = rev[k];
+  if (!ok || rv == 0xFFFFFFFFu) {
+    hard[q] = 1;
+    return;
+  }
+  hard[q] = 0;
+  if (!moved) {
+    perm_index[k] = 0;
+    perm_index[rv] = 0;
+    return;
+  }
+  const uint32_t slot = (uint32_t)(1 + 2 * (p0 + q));
+  for (uint32_t a = 0; a < B; ++a) {
+    double best = __builtin_huge_val();
+    uint32_t bc = 0;
+    for (uint32_t b = 0; b < B; ++b) {
+      const double c = C[(uint64_t)a * B + b];
+      if (c < best) {
+        best = c;
+        bc = b;
+      }
+    }
+    perms[(uint64_t)slot * B + a] = bc;          // row: column of row a
+    perms[(uint64_t)(slot + 1) * B + bc] = a;    // inverse
+  }
+  perm_index[k] = slot;
+  perm_index[rv] = slot + 1;
+}
+
+// list[h] = q for the flagged pairs (scan of the flags in `off`)
+__global__ __launch_bounds__(256) void k_pair_hard_list(uint64_t np, const uint32_t* __restrict__ hard,
+                                                        const uint32_t* __restrict__ off, uint32_t* __restrict__ list) {
+  const uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < np && hard[q]) list[off[q]] = (uint32_t)q;
+}
+
+// One wave per pair of [p0, p0 + np) (grid-stride): the exact assignment of
+// its cost matrix (copied to LDS when it fits, read from C otherwise).  LDS:
+// potentials u, v, distances minv ((B + 1) doubles each), matching p,
+// predecessors way, scanned flags ((B + 1) words each), the assignment and
+// its inverse (B words each), then the matrix.
+__global__ __launch_bounds__(SORT_BLOCK) void k_pair_lap(const double* __restrict__ Cg, uint32_t B, uint64_t p0,
+                                                         const uint32_t* __restrict__ list,
+                                                         const uint32_t* __restrict__ list_n, int cost_in_lds,
+                                                         const uint32_t* __restrict__ pk,
+                                                         const uint32_t* __restrict__ rev,
+                                                         uint32_t* __restrict__ perm_index,
+                                                         uint32_t* __restrict__ perms, uint32_t* __restrict__ bad) {
   extern __shared__ double lds[];
-  const uint32_t B = c.B, lane = threadIdx.x;
+  const uint32_t lane = threadIdx.x;
   double* u = lds;
   double* v = u + (B + 1);
   double* minv = v + (B + 1);
@@ -342,42 +445,20 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_sort_pairs(CostArgs c, const Rea
   int32_t* used = way + (B + 1);
   uint32_t* rowv = reinterpret_cast<uint32_t*>(used + (B + 1));
   uint32_t* colv = rowv + B;
-  double* C = cost_in_lds ? lds + 3 * (B + 1) + (3 * (B + 1) + 2 * B + 1) / 2 : cost_ws + (uint64_t)blockIdx.x * B * B;
-  // (rows_in_lds) the B rows of vertex i, then those of j, staged with
-  // coalesced loads: the cost loops then read LDS instead of issuing a
-  // dependent global load per element
-  const uint32_t vrow = B * c.stride;  // reals per vertex
-  Real* stage = reinterpret_cast<Real*>(C + (cost_in_lds ? (uint64_t)B * B : 0));
+  double* Cl = lds + 3 * (B + 1) + (3 * (B + 1) + 2 * B + 1) / 2;
   const double INF = __builtin_huge_val();
-  const uint64_t per = (n_keys + gridDim.x - 1) / gridDim.x;
-  const uint64_t k_lo = (uint64_t)blockIdx.x * per, k_hi = k_lo + per < n_keys ? k_lo + per : n_keys;
-  uint64_t staged_i = ~0ull;
-  for (uint64_t k = k_lo; k < k_hi; ++k) {
-    const uint64_t key = keys[k];
-    const uint64_t i = key / V;
-    const uint32_t j = nbr[k];
-    if (i > j) continue;  // block-uniform
+  const uint32_t BB = B * B;
+  const uint64_t n_list = *list_n;
+  for (uint64_t h = blockIdx.x; h < n_list; h += gridDim.x) {
+    const uint64_t q0 = list[h];
+    const double* C = Cg + q0 * BB;
+    if (cost_in_lds) {
+      for (uint32_t e = lane; e < BB; e += SORT_BLOCK) Cl[e] = C[e];
+      C = Cl;
+    }
+    const uint32_t k = pk[p0 + q0];
     const uint32_t rv = rev[k];
     const bool has_rev = rv != 0xFFFFFFFFu;  // always, the lists being symmetric
-    const uint64_t r = rv;
-    // costs
-    const Real* ri = rows + i * vrow;
-    const Real* rj = rows + (uint64_t)j * vrow;
-    if (rows_in_lds) {
-      if (i != staged_i)
-        for (uint32_t q = lane; q < vrow; q += SORT_BLOCK) stage[q] = ri[q];
-      for (uint32_t q = lane; q < vrow; q += SORT_BLOCK) stage[vrow + q] = rj[q];
-      staged_i = i;
-      __syncthreads();
-      ri = stage;
-      rj = stage + vrow;
-    }
-    bool ok = true;
-    for (uint32_t e = lane; e < B * B; e += SORT_BLOCK) {
-      const uint32_t a = e / B, b = e - a * B;
-      C[e] = pair_cost(c, ri + a * c.stride, rj + b * c.stride, ok);
-    }
-    if (!ok) atomicOr(bad, 1u);
     for (uint32_t q = lane; q <= B; q += SORT_BLOCK) {
       u[q] = 0;
       v[q] = 0;
@@ -446,7 +527,7 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_sort_pairs(CostArgs c, const Rea
       if (lane == 0) {
         atomicOr(bad, 1u);
         perm_index[k] = 0;
-        if (has_rev) perm_index[r] = 0;
+        if (has_rev) perm_index[rv] = 0;
       }
       __syncthreads();
       continue;
@@ -462,23 +543,48 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_sort_pairs(CostArgs c, const Rea
     if (!moved) {
       if (lane == 0) {
         perm_index[k] = 0;
-        perm_index[r] = 0;
+        perm_index[rv] = 0;
       }
     } else {
-      uint32_t slot = 0;
-      if (lane == 0) slot = 1 + atomicAdd(counter, 2u);
-      slot = __shfl(slot, 0);
+      // fixed slots per pair (a counter shared by every pair serialised the
+      // kernel on one address); the table is compacted afterwards
+      const uint32_t slot = (uint32_t)(1 + 2 * (p0 + q0));
       for (uint32_t q = lane; q < B; q += SORT_BLOCK) {
         perms[(uint64_t)slot * B + q] = rowv[q];
         perms[(uint64_t)(slot + 1) * B + q] = colv[q];
       }
       if (lane == 0) {
         perm_index[k] = slot;
-        perm_index[r] = slot + 1;
+        perm_index[rv] = slot + 1;
       }
     }
-    __syncthreads();  // the LDS and cost scratch are reused by the next pair
+    __syncthreads();  // the LDS is reused by the next pair
   }
+}
+
+// compaction of the table: moved[p] = 1 for pairs whose assignment is not
+// the identity, then (after a scan) their two permutations move from slots
+// 1 + 2p to 1 + 2 scan[p] and both keys are re-pointed
+__global__ __launch_bounds__(256) void k_perm_moved(uint64_t np, const uint32_t* __restrict__ pk,
+                                                    const uint32_t* __restrict__ perm_index,
+                                                    uint32_t* __restrict__ moved) {
+  const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < np) moved[p] = perm_index[pk[p]] != 0u ? 1u : 0u;
+}
+__global__ __launch_bounds__(256) void k_perm_compact(uint64_t np, uint32_t B, const uint32_t* __restrict__ pk,
+                                                      const uint32_t* __restrict__ rev,
+                                                      const uint32_t* __restrict__ scan,
+                                                      uint32_t* __restrict__ perm_index,
+                                                      const uint32_t* __restrict__ perms,
+                                                      uint32_t* __restrict__ out) {
+  const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  const uint32_t k = pk[p];
+  if (perm_index[k] == 0u) return;
+  const uint64_t from = 1 + 2 * p, to = 1 + 2 * (uint64_t)scan[p];
+  for (uint32_t q = 0; q < 2 * B; ++q) out[to * B + q] = perms[from * B + q];
+  perm_index[k] = (uint32_t)to;
+  perm_index[rev[k]] = (uint32_t)to + 1;
 }
 
 __global__ void k_identity_perm(uint32_t* perms, uint32_t B) {
@@ -582,38 +688,81 @@ extern "C" int brille_grid_sort(brille_grid* G, const brille_cost_info* values_c
   ca.vv = vc.vector_cost;
   ca.cs = cc.scalar_cost;
   ca.cv = cc.vector_cost;
-  const unsigned blocks = (unsigned)std::min<uint64_t>(std::max<uint64_t>(n_keys, 1), (uint64_t)std::max(G->n_cu, 1u) * 32);
-  // the LAP state, then the cost matrix when it fits in 48 KB of LDS
+  ca.stride = d.rows64 ? d.stride64 : d.stride32;
+  // the forward pairs (i < j) in key order
+  uint32_t* fcnt = buf.get<uint32_t>(V + 1);
+  if (!fcnt) return fail(BRILLE_ERR_DEVICE, "hipMalloc failed for the sort work space");
+  zero_words(fcnt, (uint32_t)(V + 1), s);
+  hipLaunchKernelGGL(k_pair_fwd, dim3(vb), dim3(256), 0, s, V, koff, nbr, (const uint32_t*)nullptr, fcnt,
+                     (uint32_t*)nullptr, (uint32_t*)nullptr, (uint32_t*)nullptr);
+  if (int rc = exclusive_scan_u32(fcnt, V + 1, part, s); rc != BRILLE_OK) return rc;
+  uint32_t np32 = 0;
+  if (hipMemcpyAsync(&np32, fcnt + V, 4, hipMemcpyDeviceToHost, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
+    return fail(BRILLE_ERR_DEVICE, "reading the pair count failed");
+  const uint64_t n_pairs = np32;
+  uint32_t* pi = buf.get<uint32_t>(n_pairs);
+  uint32_t* pj = buf.get<uint32_t>(n_pairs);
+  uint32_t* pk = buf.get<uint32_t>(n_pairs);
+  if (!pi || !pj || !pk) return fail(BRILLE_ERR_DEVICE, "hipMalloc failed for the sort work space");
+  hipLaunchKernelGGL(k_pair_fwd, dim3(vb), dim3(256), 0, s, V, koff, nbr, (const uint32_t*)fcnt, (uint32_t*)nullptr,
+                     pi, pj, pk);
+  // pairs in chunks whose cost matrices take at most 512 MB
+  const uint64_t BB = (uint64_t)d.B * d.B;
+  const uint64_t chunk = std::max<uint64_t>(1, std::min<uint64_t>(n_pairs, (512ull << 20) / (BB * 8)));
+  double* cost = buf.get<double>(std::max<uint64_t>(chunk, 1) * BB);
+  uint32_t* hard = buf.get<uint32_t>(chunk + 1);
+  uint32_t* hoff = buf.get<uint32_t>(chunk + 1);
+  uint32_t* hpart = buf.get<uint32_t>((chunk + 1 + 4095) / 4096 + 1);
+  uint32_t* hlist = buf.get<uint32_t>(chunk);
+  if (!cost || !hard || !hoff || !hpart || !hlist) return fail(BRILLE_ERR_DEVICE, "hipMalloc failed for the cost matrices");
   const size_t lds_state = 3 * (d.B + 1) * sizeof(double) + ((3 * (d.B + 1) + 2 * d.B + 1) / 2) * sizeof(double);
-  const size_t lds_cost = (size_t)d.B * d.B * sizeof(double);
-  const int cost_in_lds = lds_state + lds_cost <= 48 * 1024 ? 1 : 0;
-  double* cost_ws = cost_in_lds ? nullptr : buf.get<double>((uint64_t)blocks * d.B * d.B);
-  if (!cost_in_lds && !cost_ws) return fail(BRILLE_ERR_DEVICE, "hipMalloc failed for the cost matrices");
-  const size_t rs = d.rows64 ? sizeof(double) : sizeof(float);
-  const size_t lds_rows = 2 * (size_t)d.B * (d.rows64 ? d.stride64 : d.stride32) * rs;
-  const int rows_in_lds = cost_in_lds && lds_state + lds_cost + lds_rows <= 60 * 1024 ? 1 : 0;
-  const size_t lds = lds_state + (cost_in_lds ? lds_cost : 0) + (rows_in_lds ? lds_rows : 0);
-  if (n_keys) {
-    if (d.rows64) {
-      ca.stride = d.stride64;
-      hipLaunchKernelGGL(k_sort_pairs<double>, dim3(blocks), dim3(SORT_BLOCK), lds, s, ca, d.rows64, V, n_keys, keys, nbr,
-                         rev, cost_ws, cost_in_lds, rows_in_lds, pidx, perms, counter, counter + 1);
-    } else {
-      ca.stride = d.stride32;
-      hipLaunchKernelGGL(k_sort_pairs<float>, dim3(blocks), dim3(SORT_BLOCK), lds, s, ca, d.rows32, V, n_keys, keys, nbr,
-                         rev, cost_ws, cost_in_lds, rows_in_lds, pidx, perms, counter, counter + 1);
-    }
+  const int cost_in_lds = lds_state + BB * sizeof(double) <= 48 * 1024 ? 1 : 0;
+  const size_t lds = lds_state + (cost_in_lds ? BB * sizeof(double) : 0);
+  const unsigned lap_blocks = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(chunk, (uint64_t)std::max(G->n_cu, 1u) * 32));
+  for (uint64_t p0 = 0; p0 < n_pairs; p0 += chunk) {
+    const uint64_t np = std::min<uint64_t>(chunk, n_pairs - p0);
+    const unsigned cb = (unsigned)((np * BB + 255) / 256);
+    if (d.rows64)
+      hipLaunchKernelGGL(k_pair_costs<double>, dim3(cb), dim3(256), 0, s, ca, d.rows64, pi, pj, p0, np, cost, counter + 1);
+    else
+      hipLaunchKernelGGL(k_pair_costs<float>, dim3(cb), dim3(256), 0, s, ca, d.rows32, pi, pj, p0, np, cost, counter + 1);
+    const unsigned qb = (unsigned)((np + 255) / 256);
+    hipLaunchKernelGGL(k_pair_fast, dim3(qb), dim3(256), 0, s, (const double*)cost, d.B, p0, np, pk, rev, pidx, perms,
+                       hard);
+    // offsets of the flagged pairs (flags kept for the listing)
+    if (hipMemcpyAsync(hoff, hard, np * 4, hipMemcpyDeviceToDevice, s) != hipSuccess)
+      return fail(BRILLE_ERR_DEVICE, "hipMemcpyAsync failed");
+    zero_words(hoff + np, 1, s);
+    if (int rc = exclusive_scan_u32(hoff, np + 1, hpart, s); rc != BRILLE_OK) return rc;
+    hipLaunchKernelGGL(k_pair_hard_list, dim3(qb), dim3(256), 0, s, np, hard, hoff, hlist);
+    hipLaunchKernelGGL(k_pair_lap, dim3(lap_blocks), dim3(SORT_BLOCK), lds, s, (const double*)cost, d.B, p0,
+                       (const uint32_t*)hlist, (const uint32_t*)(hoff + np), cost_in_lds, pk, rev, pidx, perms,
+                       counter + 1);
   }
+  // compact: the non-identity pairs' permutations, in pair order
+  uint32_t* moved = buf.get<uint32_t>(n_pairs + 1);
+  uint32_t* ppart = buf.get<uint32_t>((n_pairs + 1 + 4095) / 4096 + 1);
+  if (!moved || !ppart) return fail(BRILLE_ERR_DEVICE, "hipMalloc failed for the sort work space");
+  zero_words(moved, (uint32_t)(n_pairs + 1), s);
+  const unsigned pb = (unsigned)((n_pairs + 255) / 256);
+  if (n_pairs) hipLaunchKernelGGL(k_perm_moved, dim3(pb), dim3(256), 0, s, n_pairs, pk, pidx, moved);
+  if (int rc = exclusive_scan_u32(moved, n_pairs + 1, ppart, s); rc != BRILLE_OK) return rc;
+  uint32_t n_moved = 0;
+  if (hipMemcpyAsync(&n_moved, moved + n_pairs, 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return fail(BRILLE_ERR_DEVICE, "sort failed");
+  uint32_t* perms_out = buf.get<uint32_t>((1 + 2 * (uint64_t)n_moved) * d.B);
+  if (!perms_out) return fail(BRILLE_ERR_DEVICE, "hipMalloc failed for the permutation table");
+  hipLaunchKernelGGL(k_identity_perm, dim3(1), dim3(256), 0, s, perms_out, d.B);
+  if (n_pairs) hipLaunchKernelGGL(k_perm_compact, dim3(pb), dim3(256), 0, s, n_pairs, d.B, pk, rev, moved, pidx,
+                                  perms, perms_out);
   if (hipGetLastError() != hipSuccess) return fail(BRILLE_ERR_DEVICE, "sort launch failed");
   uint32_t hc[2] = {0, 0};
   if (hipMemcpyAsync(hc, counter, 8, hipMemcpyDeviceToHost, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
     return fail(BRILLE_ERR_DEVICE, "sort failed");
   if (hc[1]) return fail(BRILLE_ERR_NUMERIC, "Complex Hermitian angle cos(theta) > 1 while sorting");
-  const uint32_t n_perms = 1 + hc[0];
-  uint32_t* perms_fit = buf.get<uint32_t>((uint64_t)n_perms * d.B);
-  if (!perms_fit) return fail(BRILLE_ERR_DEVICE, "hipMalloc failed for the permutation table");
-  if (hipMemcpy(perms_fit, perms, (uint64_t)n_perms * d.B * 4, hipMemcpyDeviceToDevice) != hipSuccess)
-    return fail(BRILLE_ERR_DEVICE, "hipMemcpy failed");
+  const uint32_t n_perms = 1 + 2 * n_moved;
+  uint32_t* perms_fit = perms_out;
   // install: the previous sorted table goes (after the work still queued on
   // the device that may read it), a fill-given one stays in the payload
   // allocations until the next fill
