@@ -475,7 +475,7 @@ __global__ __launch_bounds__(TILE) void k_bin_sums(GridConst g, DataConst d, Pro
   // fewer rows in flight keep the registers for occupancy
   constexpr int GRP = sizeof(Real) == 4 ? 2 : 1;
   tile_sums<Real, true, L, SC, SV, TILE, GRP>(g, d, s_rec, s_ph, s_q, s_out, 0, npts, ph_in_lds, vals, vecs, s_wide,
-                                              table, n_tab, kOpFrames ? s_frames : nullptr);
+                                              table, n_tab, kOpFrames ? s_frames : nullptr, true);
 }
 
 // The row table: thread t = (block row, mode b) phases the row of a cube's or

@@ -459,6 +459,10 @@ static int grid_fill(brille_grid* G, const brille_data_desc* vals, const brille_
     chk(upload(gamma->F0, (size_t)gamma->n_atoms * gamma->n_ops, &d.F0, A));
     chk(upload(gamma->vector_index, (size_t)gamma->n_atoms * gamma->n_ops, &d.vidx, A));
     chk(upload(gamma->vectors, 3 * (size_t)gamma->n_vectors, &d.gvec, A));
+    std::vector<double> gkey(3 * (size_t)gamma->n_atoms * gamma->n_ops);
+    for (size_t key = 0; key < (size_t)gamma->n_atoms * gamma->n_ops; ++key)
+      for (int c = 0; c < 3; ++c) gkey[3 * key + c] = gamma->vectors[3 * (size_t)gamma->vector_index[key] + c];
+    chk(upload(gkey.data(), gkey.size(), &d.gkey, A));
     if (gamma->cart_rotations) chk(upload(gamma->cart_rotations, 9 * (size_t)gamma->n_ops, &d.cart_rot, A));
   }
   if (perm && perm->n_keys) {

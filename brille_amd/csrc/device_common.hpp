@@ -107,6 +107,9 @@ struct DataConst {
   const uint32_t* F0;
   const uint32_t* vidx;
   const double* gvec;
+  // gvec[vidx[k * gamma_ops + op]] for every (atom k, op), 3 doubles each:
+  // the phase vector of a key in one load instead of two dependent ones
+  const double* gkey;
   const double* cart_rot;
   // PermutationTable
   uint64_t n_keys;

@@ -375,7 +375,7 @@ __device__ __forceinline__ void rotate_mat3(int rot_kind, int lu, const GridCons
 template <typename Real>
 __device__ inline typename Cplx<Real>::T gamma_phase(const DataConst& d, const double* q, uint32_t invrot, uint32_t k) {
   const uint32_t key = k * d.gamma_ops + invrot;
-  const double* dv = d.gvec + 3 * (uint64_t)d.vidx[key];
+  const double* dv = d.gkey ? d.gkey + 3 * (uint64_t)key : d.gvec + 3 * (uint64_t)d.vidx[key];
   double dot = 0;
   dot += q[0] * dv[0];
   dot += q[1] * dv[1];
@@ -849,10 +849,6 @@ enum Layout : int {
   LAYOUT_WIDE = 3      // as GAMMA or SCALAR with long rows (up to kWideMax complex): one wave per (point, mode)
 };
 constexpr uint32_t kWideMax = 128;  // complex elements per row the wide phase 2 handles (two per lane)
-// modes of a point one wave sums at once in the wide phase 2: two (their loads
-// in flight together) doubled the registers and took the config-2 batch from
-// 466 to 511 ms, so one
-constexpr uint32_t kWideModes = 1;
 
 __device__ __forceinline__ float wave_sum_f(float v) {
 #pragma unroll
@@ -952,122 +948,135 @@ struct WideCache {
   uint32_t dst[2];
 };
 
-// Phase 2 for rows too long for one lane: U consecutive modes b0 .. b0 + U - 1
-// of one point by one wave (modes past B are skipped).  Lane l takes the row's
-// complex elements l and l + 64 (TWO: rows of more than 64), so every row is
-// read with coalesced loads; the slot rows and antiphase factors (from the
-// wide factor table, or Hermitian products across the wave) of all U modes
-// are requested before any is used.  Each mode is then summed, rotated
-// through the wave's LDS slice `lds` (the Gamma rotation mixes the three
-// elements of an atom) and written; `wc` keeps the point's rotation data
-// while the wave stays on it (`fresh`: the wave's previous point was another).
-template <typename Real, int U, bool TWO>
-__device__ inline void units_wide(const GridConst& g, const DataConst& d, const PointRec<Real>& r, const double* q,
-                                  uint32_t b0, Real* ov_point, typename Cplx<Real>::T* oc_point,
-                                  typename Cplx<Real>::T* lds, bool gamma, bool cart, WideCache<Real>& wc,
-                                  bool fresh, bool table) {
+// The slot rows (lane l: complex elements l and l + 64 when TWO, value l) and
+// antiphase factors (wide factor table when `tab`, else Hermitian products
+// across the wave against the pivot row) of mode b of a point: what the sums
+// of one wide unit read.  A point sharing the table block (trow) of the
+// previous one has the same rows and factors.
+template <typename Real, bool TWO>
+struct WideRows {
+  typename Cplx<Real>::T v0[8], v1[TWO ? 8 : 1], e[8];
+  Real vv[8];
+};
+
+template <typename Real, bool TWO>
+__device__ __forceinline__ void wide_load(const DataConst& d, const PointRec<Real>& r, uint32_t b, bool tab,
+                                          WideRows<Real, TWO>& w) {
   using C = typename Cplx<Real>::T;
   const uint32_t l = threadIdx.x & 63u;
   const uint32_t sc = d.sc, sv = d.sv, B = d.B;
   const bool h0 = l < sc, h1 = TWO && l + 64u < sc;
   const C zero = Elem<C>::zero();
-  C v0[U][8], v1[TWO ? U : 1][8];
-  Real vv[U][8];
-  C e[U][8];
-  const bool tab = table && r.trow != kNoCube;
+  const bool t = tab && r.trow != kNoCube;
   const C* fac = wide_factors<Real>(d);
 #pragma unroll
-  for (int k = 0; k < U; ++k) {
-    const uint32_t b = b0 + (uint32_t)k < B ? b0 + (uint32_t)k : b0;  // modes past B repeat b0 (not stored)
+  for (int x = 0; x < 8; ++x) {
+    const Real* row = row_at<Real>(d, r.idx[x], b);
+    const C* rc = reinterpret_cast<const C*>(row);
+    w.v0[x] = h0 ? rc[l] : zero;
+    if constexpr (TWO) w.v1[x] = h1 ? rc[l + 64] : zero;
+    w.vv[x] = l < sv ? row[2 * sc + l] : (Real)0;
+    if (t) w.e[x] = x < (int)r.nslot ? fac[r.trow + (uint32_t)x * B + b] : C{(Real)1, (Real)0};
+  }
+  if (!t) {
+    const C* prow = reinterpret_cast<const C*>(row_at<Real>(d, r.pivot, b));
+    const C p0 = h0 ? prow[l] : zero;
+    C p1 = zero;
+    if constexpr (TWO) p1 = h1 ? prow[l + 64] : zero;
+    if constexpr (TWO) {
+      wide_antiphases<Real, 8>(p0, p1, w.v0, w.v1, w.e, sc);
+    } else {
+      C z1[8];
 #pragma unroll
-    for (int x = 0; x < 8; ++x) {
-      const Real* row = row_at<Real>(d, r.idx[x], b);
-      const C* rc = reinterpret_cast<const C*>(row);
-      v0[k][x] = h0 ? rc[l] : zero;
-      if constexpr (TWO) v1[k][x] = h1 ? rc[l + 64] : zero;
-      vv[k][x] = l < sv ? row[2 * sc + l] : (Real)0;
-      if (tab) e[k][x] = x < (int)r.nslot ? fac[r.trow + (uint32_t)x * B + b] : C{(Real)1, (Real)0};
+      for (int x = 0; x < 8; ++x) z1[x] = zero;
+      wide_antiphases<Real, 8>(p0, zero, w.v0, z1, w.e, sc);
     }
   }
-  if (!tab) {
-#pragma unroll
-    for (int k = 0; k < U; ++k) {
-      const uint32_t b = b0 + (uint32_t)k < B ? b0 + (uint32_t)k : b0;
-      const C* prow = reinterpret_cast<const C*>(row_at<Real>(d, r.pivot, b));
-      const C p0 = h0 ? prow[l] : zero;
-      C p1 = zero;
-      if constexpr (TWO) p1 = h1 ? prow[l + 64] : zero;
-      if constexpr (TWO) {
-        wide_antiphases<Real, 8>(p0, p1, v0[k], v1[k], e[k], sc);
-      } else {
-        C z1[8];
-#pragma unroll
-        for (int x = 0; x < 8; ++x) z1[x] = zero;
-        wide_antiphases<Real, 8>(p0, zero, v0[k], z1, e[k], sc);
-      }
-    }
-  }
+}
+
+// The sums of mode b of a point from its loaded rows, the Gamma rotation
+// through the wave's LDS slice `lds` (it mixes the three elements of an
+// atom), and the stores.  `wc` keeps the point's rotation data while
+// `fresh` is false (the previous unit was the same point's).
+template <typename Real, bool TWO>
+__device__ __forceinline__ void wide_sum_store(const GridConst& g, const DataConst& d, const PointRec<Real>& r,
+                                               const double* q, uint32_t b, const WideRows<Real, TWO>& w,
+                                               Real* ov_point, typename Cplx<Real>::T* oc_point,
+                                               typename Cplx<Real>::T* lds, bool gamma, bool cart,
+                                               WideCache<Real>& wc, bool fresh) {
+  using C = typename Cplx<Real>::T;
+  const uint32_t l = threadIdx.x & 63u;
+  const uint32_t sc = d.sc, sv = d.sv;
+  const bool h0 = l < sc, h1 = TWO && l + 64u < sc;
+  const C zero = Elem<C>::zero();
   typedef Real real2 __attribute__((ext_vector_type(2)));
+  C a0 = zero, a1 = zero;
+  Real va = 0;
 #pragma unroll
-  for (int k = 0; k < U; ++k) {
-    const uint32_t b = b0 + (uint32_t)k;
-    if (b >= B) break;  // wave-uniform
-    C a0 = zero, a1 = zero;
-    Real va = 0;
-#pragma unroll
-    for (int x = 0; x < 8; ++x) {
-      const Real w = r.w[x];
-      Real px, py;
-      phase_elem<Real>(e[k][x], v0[k][x].x, v0[k][x].y, &px, &py);
-      a0.x = fma_r(w, px, a0.x);
-      a0.y = fma_r(w, py, a0.y);
-      if constexpr (TWO) {
-        phase_elem<Real>(e[k][x], v1[k][x].x, v1[k][x].y, &px, &py);
-        a1.x = fma_r(w, px, a1.x);
-        a1.y = fma_r(w, py, a1.y);
-      }
-      va = fma_r(w, vv[k][x], va);
+  for (int x = 0; x < 8; ++x) {
+    const Real wx = r.w[x];
+    Real px, py;
+    phase_elem<Real>(w.e[x], w.v0[x].x, w.v0[x].y, &px, &py);
+    a0.x = fma_r(wx, px, a0.x);
+    a0.y = fma_r(wx, py, a0.y);
+    if constexpr (TWO) {
+      phase_elem<Real>(w.e[x], w.v1[x].x, w.v1[x].y, &px, &py);
+      a1.x = fma_r(wx, px, a1.x);
+      a1.y = fma_r(wx, py, a1.y);
     }
-    Real* ov = ov_point + (uint64_t)b * sv;
-    C* oc = oc_point + (uint64_t)b * sc;
-    if (l < sv) __builtin_nontemporal_store(va, ov + l);
-    if (!gamma) {
-      if (h0) __builtin_nontemporal_store(real2{a0.x, a0.y}, reinterpret_cast<real2*>(oc + l));
-      if (h1) __builtin_nontemporal_store(real2{a1.x, a1.y}, reinterpret_cast<real2*>(oc + l + 64));
-      continue;
-    }
-    // rip_gamma_complex  src/interpolator_gamma.tpp:100-114, element by element
-    if (h0) lds[l] = a0;
-    if (h1) lds[l + 64] = a1;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-#pragma unroll
-    for (int h = 0; h < (TWO ? 2 : 1); ++h) {
-      const uint32_t el = l + 64u * h;
-      if (el < sc) {
-        const uint32_t a = el / 3u, i = el - 3u * a;
-        if (fresh) {
-#pragma unroll
-          for (int j = 0; j < 3; ++j)
-            wc.R[h][j] = cart ? (Real)d.cart_rot[9 * (uint64_t)r.invrot + 3 * i + j]
-                              : (Real)g.rot[9 * (uint64_t)r.invrot + 3 * i + j];
-          wc.ph[h] = gamma_phase<Real>(d, q, r.invrot, a);
-          wc.dst[h] = 3u * d.F0[a * d.gamma_ops + r.invrot] + i;
-        }
-        const C v[3] = {lds[3 * a], lds[3 * a + 1], lds[3 * a + 2]};
-        C t = zero;
-#pragma unroll
-        for (int j = 0; j < 3; ++j) t = Elem<C>::add(t, Elem<C>::scale(v[j], wc.R[h][j]));
-        const C o = cmul(wc.ph[h], t);
-        __builtin_nontemporal_store(real2{o.x, o.y}, reinterpret_cast<real2*>(oc + wc.dst[h]));
-      }
-    }
-    fresh = false;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    va = fma_r(wx, w.vv[x], va);
   }
+  Real* ov = ov_point + (uint64_t)b * sv;
+  C* oc = oc_point + (uint64_t)b * sc;
+  if (l < sv) __builtin_nontemporal_store(va, ov + l);
+  if (!gamma) {
+    if (h0) __builtin_nontemporal_store(real2{a0.x, a0.y}, reinterpret_cast<real2*>(oc + l));
+    if (h1) __builtin_nontemporal_store(real2{a1.x, a1.y}, reinterpret_cast<real2*>(oc + l + 64));
+    return;
+  }
+  // rip_gamma_complex  src/interpolator_gamma.tpp:100-114, element by element
+  if (h0) lds[l] = a0;
+  if (h1) lds[l + 64] = a1;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+  for (int h = 0; h < (TWO ? 2 : 1); ++h) {
+    const uint32_t el = l + 64u * h;
+    if (el < sc) {
+      const uint32_t a = el / 3u, i = el - 3u * a;
+      if (fresh) {
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+          wc.R[h][j] = cart ? (Real)d.cart_rot[9 * (uint64_t)r.invrot + 3 * i + j]
+                            : (Real)g.rot[9 * (uint64_t)r.invrot + 3 * i + j];
+        wc.ph[h] = gamma_phase<Real>(d, q, r.invrot, a);
+        wc.dst[h] = 3u * d.F0[a * d.gamma_ops + r.invrot] + i;
+      }
+      const C v[3] = {lds[3 * a], lds[3 * a + 1], lds[3 * a + 2]};
+      C t = zero;
+#pragma unroll
+      for (int j = 0; j < 3; ++j) t = Elem<C>::add(t, Elem<C>::scale(v[j], wc.R[h][j]));
+      const C o = cmul(wc.ph[h], t);
+      __builtin_nontemporal_store(real2{o.x, o.y}, reinterpret_cast<real2*>(oc + wc.dst[h]));
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Phase 2 for rows too long for one lane: mode b of one point by one wave.
+// Lane l takes the row's complex elements l and l + 64 (TWO: rows of more
+// than 64), so every row is read with coalesced loads.
+template <typename Real, bool TWO>
+__device__ inline void unit_wide(const GridConst& g, const DataConst& d, const PointRec<Real>& r, const double* q,
+                                 uint32_t b, Real* ov_point, typename Cplx<Real>::T* oc_point,
+                                 typename Cplx<Real>::T* lds, bool gamma, bool cart, WideCache<Real>& wc, bool fresh,
+                                 bool table) {
+  WideRows<Real, TWO> w;
+  wide_load<Real, TWO>(d, r, b, table, w);
+  wide_sum_store<Real, TWO>(g, d, r, q, b, w, ov_point, oc_point, lds, gamma, cart, wc, fresh);
 }
 
 struct LaunchArgs {
@@ -1298,7 +1307,7 @@ __device__ __forceinline__ void tile_sums(const GridConst& g, const DataConst& d
                                           const uint32_t* out_row, uint64_t row0, uint32_t npts, bool ph_in_lds,
                                           void* vals_, void* vecs_, typename Cplx<Real>::T* s_wide = nullptr,
                                           bool table = false, uint32_t n_coop = 0,
-                                          const GammaFrame<Real>* op_frames = nullptr) {
+                                          const GammaFrame<Real>* op_frames = nullptr, bool node_order = false) {
   using C = typename Cplx<Real>::T;
   constexpr int NPH = TileShape<L, SC>::NPH;
   const uint32_t B = d.B;
@@ -1307,33 +1316,70 @@ __device__ __forceinline__ void tile_sums(const GridConst& g, const DataConst& d
   C* vecs = (C*)vecs_;
   const bool cart = d.vec_lu == BRILLE_LU_ANGSTROM;
   if constexpr (L == LAYOUT_WIDE) {
-    // one wave per kWideModes consecutive modes of a point; wave-uniform loop
-    constexpr uint32_t U = kWideModes;
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
     const bool gamma = d.vec_rot == BRILLE_ROT_GAMMA && d.ec[1] > 0;
     C* lds = s_wide + wave * kWideMax;
     WideCache<Real> wc;
-    uint32_t last = 0xFFFFFFFFu;
-    const uint32_t CB = (B + U - 1) / U;  // mode chunks per point
-    const uint32_t chunks = npts * CB;
     const bool two = d.sc > 64;
     const bool tab = wide_factors<Real>(d) != nullptr;
-    for (uint32_t c = wave; c < chunks; c += TB / 64) {
-      const uint32_t pl = c / CB, b0 = (c - pl * CB) * U;
-      const PointRec<Real>& r = s_rec[pl];
-      const uint64_t row = ((out_row ? (uint64_t)out_row[pl] : row0 + pl)) * B;
-      Real* ov = vals + row * d.sv;
-      C* oc = vecs + row * d.sc;
-      if (r.nslot == 0) {
-        const uint32_t b1 = b0 + U < B ? b0 + U : B;
-        for (uint32_t e = lane; e < (b1 - b0) * d.sv; e += 64) ov[(uint64_t)b0 * d.sv + e] = 0;
-        for (uint32_t e = lane; e < (b1 - b0) * d.sc; e += 64) oc[(uint64_t)b0 * d.sc + e] = Elem<C>::zero();
-        continue;
+    if (!node_order) {
+      // Point-major: wave w takes units w, w + TB/64, ... of (point, mode),
+      // so it stays on a point for its share of the modes and keeps the
+      // point's rotation data (a second mode in flight per wave doubled the
+      // registers and made the config-2 batch slower, 466 -> 511 ms)
+      uint32_t last = 0xFFFFFFFFu;
+      for (uint32_t c = wave; c < units; c += TB / 64) {
+        const uint32_t pl = c / B, b = c - pl * B;
+        const PointRec<Real>& r = s_rec[pl];
+        const uint64_t row = ((out_row ? (uint64_t)out_row[pl] : row0 + pl)) * B;
+        Real* ov = vals + row * d.sv;
+        C* oc = vecs + row * d.sc;
+        if (r.nslot == 0) {
+          for (uint32_t e = lane; e < d.sv; e += 64) ov[(uint64_t)b * d.sv + e] = 0;
+          for (uint32_t e = lane; e < d.sc; e += 64) oc[(uint64_t)b * d.sc + e] = Elem<C>::zero();
+          continue;
+        }
+        if (two) unit_wide<Real, true>(g, d, r, s_q + 3 * pl, b, ov, oc, lds, gamma, cart, wc, pl != last, tab);
+        else unit_wide<Real, false>(g, d, r, s_q + 3 * pl, b, ov, oc, lds, gamma, cart, wc, pl != last, tab);
+        last = pl;
       }
-      if (two) units_wide<Real, U, true>(g, d, r, s_q + 3 * pl, b0, ov, oc, lds, gamma, cart, wc, pl != last, tab);
-      else units_wide<Real, U, false>(g, d, r, s_q + 3 * pl, b0, ov, oc, lds, gamma, cart, wc, pl != last, tab);
-      last = pl;
+      return;
     }
+    // Mode-major (node-ordered tiles): wave w takes modes w, w + TB/64, ...
+    // and for each walks the tile's points in their node order.  Consecutive
+    // points of one table block (cube or tetrahedron, same trow) have the
+    // same slot rows and factors, which stay in the wave's registers; they
+    // are loaded again only when the block changes.  Wave-uniform loops.
+    // (The point's rotation data are recomputed for every mode here: keeping
+    // them for the tile's points would need an LDS table per point and atom.
+    // Fetching them one point ahead saved nothing, 364 -> 370 ms for the
+    // config-2 batch: the cost is the loads' issue, not their latency.)
+    auto walk = [&](auto two_tag) {
+      constexpr bool TWO = decltype(two_tag)::value;
+      WideRows<Real, TWO> w;
+      for (uint32_t b = wave; b < B; b += TB / 64) {
+        uint32_t cached = kNoCube;
+        for (uint32_t pl = 0; pl < npts; ++pl) {
+          const PointRec<Real>& r = s_rec[pl];
+          const uint64_t row = ((out_row ? (uint64_t)out_row[pl] : row0 + pl)) * B;
+          Real* ov = vals + row * d.sv;
+          C* oc = vecs + row * d.sc;
+          if (r.nslot == 0) {
+            for (uint32_t e = lane; e < d.sv; e += 64) ov[(uint64_t)b * d.sv + e] = 0;
+            for (uint32_t e = lane; e < d.sc; e += 64) oc[(uint64_t)b * d.sc + e] = Elem<C>::zero();
+            continue;
+          }
+          const bool reuse = tab && r.trow != kNoCube && r.trow == cached;
+          if (!reuse) {
+            wide_load<Real, TWO>(d, r, b, tab, w);
+            cached = (tab && r.trow != kNoCube) ? r.trow : kNoCube;
+          }
+          wide_sum_store<Real, TWO>(g, d, r, s_q + 3 * pl, b, w, ov, oc, lds, gamma, cart, wc, true);
+        }
+      }
+    };
+    if (two) walk(std::true_type{});
+    else walk(std::false_type{});
     return;
   }
   // Gamma frames: per point in s_q, or per point-group operation in op_frames
