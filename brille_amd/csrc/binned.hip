@@ -385,7 +385,12 @@ __global__ __launch_bounds__(TILE) void k_bin_sums(GridConst g, DataConst d, Pro
   using C = typename Cplx<Real>::T;
   using S = TileShape<L, SC, TILE, Real>;
   __shared__ PointRec<Real> s_rec[TILE];
-  __shared__ C s_ph[TILE * S::NPH];
+  // wide rows: the phase table of the node-ordered Gamma sums instead
+  constexpr bool kWide = L == LAYOUT_WIDE;
+  constexpr uint32_t kPh = kWide ? kWidePhaseBytes / sizeof(C) : TILE * S::NPH;
+  __shared__ C s_ph[kPh];
+  __shared__ Real s_wrot[kWide ? 9 * kMaxOps : 1];
+  __shared__ uint8_t s_wdst[kWide ? kWideAtoms * kMaxOps : 1];
   // Gamma layout: one frame per point-group operation instead of one per
   // point (12 KB less LDS in FP32, 20 KB in FP64)
   constexpr bool kOpFrames = L == LAYOUT_GAMMA;
@@ -405,6 +410,14 @@ __global__ __launch_bounds__(TILE) void k_bin_sums(GridConst g, DataConst d, Pro
   const bool want_phase = d.vec_rot == BRILLE_ROT_GAMMA && d.ec[1] > 0;
   const uint32_t n_atoms = d.ec[1] / 3u;
   const bool ph_in_lds = L == LAYOUT_GAMMA || (want_phase && n_atoms <= 4);
+  WideGammaLds<Real> wg;
+  if (kWide && want_phase) {
+    wg.ph = s_ph;
+    wg.rot = s_wrot;
+    wg.dst = s_wdst;
+    wg.cap = kPh;
+    wide_gamma_ops<Real, TILE>(g, d, wg);
+  }
   __shared__ uint32_t s_wave_tab[TILE / 64];
   if (Prop) __syncthreads();
   const bool live = p < n;
@@ -475,7 +488,7 @@ __global__ __launch_bounds__(TILE) void k_bin_sums(GridConst g, DataConst d, Pro
   // fewer rows in flight keep the registers for occupancy
   constexpr int GRP = sizeof(Real) == 4 ? 2 : 1;
   tile_sums<Real, true, L, SC, SV, TILE, GRP>(g, d, s_rec, s_ph, s_q, s_out, 0, npts, ph_in_lds, vals, vecs, s_wide,
-                                              table, n_tab, kOpFrames ? s_frames : nullptr, true);
+                                              table, n_tab, kOpFrames ? s_frames : nullptr, true, wg);
 }
 
 // The row table: thread t = (block row, mode b) phases the row of a cube's or

@@ -849,6 +849,12 @@ enum Layout : int {
   LAYOUT_WIDE = 3      // as GAMMA or SCALAR with long rows (up to kWideMax complex): one wave per (point, mode)
 };
 constexpr uint32_t kWideMax = 128;  // complex elements per row the wide phase 2 handles (two per lane)
+constexpr uint32_t kWideAtoms = kWideMax / 3;  // atoms of the longest wide Gamma row
+// LDS of the node-ordered wide sums' Gamma table: per-point phases of a chunk
+// of the tile's points, every atom (FP32: 4608 complex, 192 points of 24
+// atoms).  Those sums hold over 128 VGPRs, so two blocks per CU is their
+// occupancy with or without it.
+constexpr uint32_t kWidePhaseBytes = 36864;
 
 __device__ __forceinline__ float wave_sum_f(float v) {
 #pragma unroll
@@ -1064,6 +1070,32 @@ __device__ __forceinline__ void wide_sum_store(const GridConst& g, const DataCon
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// The Gamma data of the node-ordered wide sums in LDS: phases e^{i 2 pi q.d}
+// of a chunk of the tile's points (point-major, `cap` complex), and per
+// point-group operation the rotation rows (9 per operation) and the output
+// atom of every source atom.  All null: the sums compute them per unit.
+template <typename Real>
+struct WideGammaLds {
+  typename Cplx<Real>::T* ph = nullptr;
+  Real* rot = nullptr;
+  uint8_t* dst = nullptr;
+  uint32_t cap = 0;
+};
+
+// Fill the per-operation part of WideGammaLds (operations invrot < G); the
+// caller synchronises the block before the sums read it.
+template <typename Real, int TB>
+__device__ __forceinline__ void wide_gamma_ops(const GridConst& g, const DataConst& d, const WideGammaLds<Real>& t) {
+  const bool cart = d.vec_lu == BRILLE_LU_ANGSTROM;
+  const uint32_t na = d.sc / 3u;
+  for (uint32_t k = threadIdx.x; k < 9u * g.G; k += TB)
+    t.rot[k] = cart ? (Real)d.cart_rot[k] : (Real)g.rot[k];
+  for (uint32_t k = threadIdx.x; k < na * g.G; k += TB) {
+    const uint32_t op = k / na, a = k - op * na;
+    t.dst[k] = (uint8_t)d.F0[a * d.gamma_ops + op];
+  }
 }
 
 // Phase 2 for rows too long for one lane: mode b of one point by one wave.
@@ -1307,7 +1339,8 @@ __device__ __forceinline__ void tile_sums(const GridConst& g, const DataConst& d
                                           const uint32_t* out_row, uint64_t row0, uint32_t npts, bool ph_in_lds,
                                           void* vals_, void* vecs_, typename Cplx<Real>::T* s_wide = nullptr,
                                           bool table = false, uint32_t n_coop = 0,
-                                          const GammaFrame<Real>* op_frames = nullptr, bool node_order = false) {
+                                          const GammaFrame<Real>* op_frames = nullptr, bool node_order = false,
+                                          WideGammaLds<Real> wg = {}) {
   using C = typename Cplx<Real>::T;
   constexpr int NPH = TileShape<L, SC>::NPH;
   const uint32_t B = d.B;
@@ -1350,31 +1383,58 @@ __device__ __forceinline__ void tile_sums(const GridConst& g, const DataConst& d
     // points of one table block (cube or tetrahedron, same trow) have the
     // same slot rows and factors, which stay in the wave's registers; they
     // are loaded again only when the block changes.  Wave-uniform loops.
-    // (The point's rotation data are recomputed for every mode here: keeping
-    // them for the tile's points would need an LDS table per point and atom.
-    // Fetching them one point ahead saved nothing, 364 -> 370 ms for the
-    // config-2 batch: the cost is the loads' issue, not their latency.)
+    // With `wg`, the points go in chunks whose Gamma phases the block first
+    // computes into LDS (once per point and atom instead of once per mode),
+    // and the rotation rows and atom map come from per-operation LDS tables.
+    const bool pre = gamma && wg.ph != nullptr;
+    const uint32_t na = d.sc / 3u;
+    const uint32_t chunk = pre ? (wg.cap / na < npts ? wg.cap / na : npts) : npts;
     auto walk = [&](auto two_tag) {
       constexpr bool TWO = decltype(two_tag)::value;
       WideRows<Real, TWO> w;
-      for (uint32_t b = wave; b < B; b += TB / 64) {
-        uint32_t cached = kNoCube;
-        for (uint32_t pl = 0; pl < npts; ++pl) {
-          const PointRec<Real>& r = s_rec[pl];
-          const uint64_t row = ((out_row ? (uint64_t)out_row[pl] : row0 + pl)) * B;
-          Real* ov = vals + row * d.sv;
-          C* oc = vecs + row * d.sc;
-          if (r.nslot == 0) {
-            for (uint32_t e = lane; e < d.sv; e += 64) ov[(uint64_t)b * d.sv + e] = 0;
-            for (uint32_t e = lane; e < d.sc; e += 64) oc[(uint64_t)b * d.sc + e] = Elem<C>::zero();
-            continue;
+      for (uint32_t c0 = 0; c0 < npts; c0 += chunk) {
+        const uint32_t c1 = c0 + chunk < npts ? c0 + chunk : npts;
+        if (pre) {
+          __syncthreads();  // the previous chunk's phases are read
+          for (uint32_t k = threadIdx.x; k < (c1 - c0) * na; k += TB) {
+            const uint32_t pl = c0 + k / na, a = k - (k / na) * na;
+            const PointRec<Real>& r = s_rec[pl];
+            wg.ph[k] = r.nslot ? gamma_phase<Real>(d, s_q + 3 * pl, r.invrot, a) : Elem<C>::zero();
           }
-          const bool reuse = tab && r.trow != kNoCube && r.trow == cached;
-          if (!reuse) {
-            wide_load<Real, TWO>(d, r, b, tab, w);
-            cached = (tab && r.trow != kNoCube) ? r.trow : kNoCube;
+          __syncthreads();
+        }
+        for (uint32_t b = wave; b < B; b += TB / 64) {
+          uint32_t cached = kNoCube;
+          for (uint32_t pl = c0; pl < c1; ++pl) {
+            const PointRec<Real>& r = s_rec[pl];
+            const uint64_t row = ((out_row ? (uint64_t)out_row[pl] : row0 + pl)) * B;
+            Real* ov = vals + row * d.sv;
+            C* oc = vecs + row * d.sc;
+            if (r.nslot == 0) {
+              for (uint32_t e = lane; e < d.sv; e += 64) ov[(uint64_t)b * d.sv + e] = 0;
+              for (uint32_t e = lane; e < d.sc; e += 64) oc[(uint64_t)b * d.sc + e] = Elem<C>::zero();
+              continue;
+            }
+            const bool reuse = tab && r.trow != kNoCube && r.trow == cached;
+            if (!reuse) {
+              wide_load<Real, TWO>(d, r, b, tab, w);
+              cached = (tab && r.trow != kNoCube) ? r.trow : kNoCube;
+            }
+            if (pre) {
+#pragma unroll
+              for (int h = 0; h < (TWO ? 2 : 1); ++h) {
+                const uint32_t el = lane + 64u * h;
+                if (el < d.sc) {
+                  const uint32_t a = el / 3u, i = el - 3u * a;
+                  wc.ph[h] = wg.ph[(pl - c0) * na + a];
+#pragma unroll
+                  for (int j = 0; j < 3; ++j) wc.R[h][j] = wg.rot[9u * r.invrot + 3u * i + j];
+                  wc.dst[h] = 3u * wg.dst[r.invrot * na + a] + i;
+                }
+              }
+            }
+            wide_sum_store<Real, TWO>(g, d, r, s_q + 3 * pl, b, w, ov, oc, lds, gamma, cart, wc, !pre);
           }
-          wide_sum_store<Real, TWO>(g, d, r, s_q + 3 * pl, b, w, ov, oc, lds, gamma, cart, wc, true);
         }
       }
     };

@@ -114,12 +114,18 @@ def test_binned_several_passes():
     np.testing.assert_array_equal(cb2, cb)
 
 
-def test_wide_rows_binned_equals_tile_and_oracle():
-    """A many-atom payload (rows of more than 12 complex elements: one wave per
-    (point, mode), antiphases from the wide factor table in the binned path)."""
+@pytest.mark.parametrize("which", ["fewest", "most"])
+def test_wide_rows_binned_equals_tile_and_oracle(which):
+    """Many-atom payloads (rows of more than 12 complex elements: one wave per
+    (point, mode), antiphases from the wide factor table in the binned path):
+    the wide grid of fewest atoms, and the one of most (24 atoms: the binned
+    sums take its tiles' Gamma phases in several chunks of points)."""
     grids, Q, offsets = S.batch_grids(n_grids=6, v_range=(800, 2000), n_q=20_000)
     sc = [int(sum(g.tables()["vectors_elements"])) for g in grids]
-    gi = int(np.argmin([c if c > 12 else 1 << 30 for c in sc]))
+    if which == "fewest":
+        gi = int(np.argmin([c if c > 12 else 1 << 30 for c in sc]))
+    else:
+        gi = int(np.argmax(sc))
     assert sc[gi] > 12, sc
     t = grids[gi].tables()
     say(f"wide rows: {sc[gi]} complex elements per mode, {int(t['branches'])} modes")
@@ -130,7 +136,7 @@ def test_wide_rows_binned_equals_tile_and_oracle():
         assert rb == 0 and rt == 0
         np.testing.assert_array_equal(vb, vt)
         np.testing.assert_array_equal(cb, ct)
-    m = 4000
+    m = 4000 if which == "fewest" else 1000
     rc, ov, oc, _, _ = O.ir_interpolate_at(t, Q[:m], nthreads=8)
     assert rc == 0
     assert np.max(np.abs(vb[:m] - ov)) <= 1e-12 * np.max(np.abs(ov))
