@@ -306,11 +306,13 @@ def batch_leg(args, capi, S, with_baselines):
     counts = np.diff(offsets.astype(np.int64))
     alg = float((counts * (24 + np.array(per_q_out))).sum()) + table_total
     achieved = alg / (np.mean(kms) * 1e-3) / 1e9
+    per_q = measured_traffic(2, "batch fp32", summary="batch") if n_q == 10_000_000 else None
     res = dict(value=n_q / wall * 1e3, unit="Q-points/s", ms_per_step=wall, batch_ms=float(np.mean(kms)),
                n_q=n_q, vertices_total=int(sum(V)), vertices=[min(V), max(V)], modes=[min(Bm), max(Bm)],
                schedule=sch, output_gb=float((counts * np.array(per_q_out)).sum() / 1e9),
                roofline=dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
-                             frac=achieved / HBM_PEAK_GBS,
+                             frac=achieved / HBM_PEAK_GBS, traffic=(per_q * n_q if per_q else None),
+                             traffic_bytes_per_q=per_q,
                              kernel="k_batch<float> graph replay, then the binned pipelines of the grids "
                                     "given at least binned_min points (HIP events around both)"))
     log(f"batched fp32: {wall:.3f} ms/step wall, batch {np.mean(kms):.3f} ms, {achieved:.0f} GB/s")
@@ -422,12 +424,12 @@ def sort_leg(args, capi, S, T):
     return res
 
 
-def measured_traffic(cfg, path_name):
-    """HBM bytes per Q of one call of the headline path, from the committed
-    rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE of every kernel of the
-    call, scaled by the copy-kernel calibration, over Q); None when there is no
-    summary for this config and path."""
-    path = os.path.join(ROOT, "profiles", f"hbm_traffic_binned_config{cfg}.md")
+def measured_traffic(cfg, path_name, summary="binned"):
+    """HBM bytes per Q of one call of a path, from the committed rocprofv3
+    --pmc summary (FETCH_SIZE + WRITE_SIZE of every kernel of the call, scaled
+    by the copy-kernel calibration, over Q); None when there is no summary for
+    this config and path."""
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_{summary}_config{cfg}.md")
     if not os.path.exists(path):
         return None
     key = "traffic_bytes_per_q[" + path_name + "]:"

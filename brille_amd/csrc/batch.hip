@@ -191,6 +191,13 @@ struct brille_batch {
   uint32_t n_cu = 0;
   Schedule sched;
   hipStream_t cap_stream = nullptr;
+  // streams the binned grids spread over beside the caller's (created on
+  // first use, default priority), and the events that fork them from the
+  // caller's stream and join them back
+  static constexpr int kLanes = 3;
+  hipStream_t lanes[kLanes] = {};
+  hipEvent_t fork = nullptr;
+  hipEvent_t join[kLanes] = {};
   // instantiated graphs by (precision, tile, blocks, grab)
   std::map<std::tuple<int, uint32_t, uint32_t, uint32_t>, hipGraphExec_t> graphs;
   // host copy of d_grids, refreshed when a grid's tables change, and the
@@ -379,21 +386,57 @@ static int run_with(brille_batch* b, const double* Q, const uint64_t* offsets, c
 }
 
 // The large grids after the graph (whose reset clears every grid's error
-// record), each a binned call on the same stream; errors are read by the
-// caller as for the others.
+// record), each a binned call; errors are read by the caller as for the
+// others.  One grid's call leaves most of the GPU idle at its start and end
+// (and all of it but a few blocks for a grid of few tiles), so the calls
+// spread over the caller's stream and the batch's lanes, largest first onto
+// the least loaded (by output bytes), and the caller's stream waits for all.
 static int launch_binned_grids(brille_batch* b, const double* Q, const uint64_t* offsets, const brille_run_opts& o,
                                void* const* vals, void* const* vecs, hipStream_t s, const Schedule& sc) {
-  int rc = BRILLE_OK;
   brille_run_opts ob = o;
   ob.kernel = BRILLE_KERNEL_BINNED;
   ob.check = 0;
+  std::vector<std::pair<double, uint32_t>> big;  // (output bytes, grid)
   for (size_t g = 0; g < b->grids.size(); ++g) {
     const uint64_t n = offsets[g + 1] - offsets[g];
     if (n == 0 || n < sc.binned_min) continue;
-    rc = run_interpolate(b->grids[g], Q + 3 * offsets[g], n, &ob, vals[g], vecs[g], nullptr, s, nullptr, 0);
-    if (rc != BRILLE_OK) return fail(rc, "grid " + std::to_string(g) + ": " + std::string(g_last_error_text()));
+    const DataConst& d = b->grids[g]->d;
+    big.push_back({(double)n * d.B * (d.sv + 2.0 * d.sc), (uint32_t)g});
   }
-  return BRILLE_OK;
+  if (big.empty()) return BRILLE_OK;
+  std::stable_sort(big.begin(), big.end(), [](const auto& x, const auto& y) { return x.first > y.first; });
+  const int n_lanes = big.size() > 1 ? brille_batch::kLanes : 0;
+  if (n_lanes) {
+    for (int k = 0; k < n_lanes; ++k)
+      if (!b->lanes[k]) {
+        BATCH_TRY(hipStreamCreateWithFlags(&b->lanes[k], hipStreamNonBlocking));
+        BATCH_TRY(hipEventCreateWithFlags(&b->join[k], hipEventDisableTiming));
+      }
+    if (!b->fork) BATCH_TRY(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+    BATCH_TRY(hipEventRecord(b->fork, s));
+    for (int k = 0; k < n_lanes; ++k) BATCH_TRY(hipStreamWaitEvent(b->lanes[k], b->fork, 0));
+  }
+  double load[brille_batch::kLanes + 1] = {};
+  int rc = BRILLE_OK;
+  for (const auto& [bytes, g] : big) {
+    int k = 0;
+    for (int j = 1; j <= n_lanes; ++j)
+      if (load[j] < load[k]) k = j;
+    load[k] += bytes;
+    hipStream_t st = k == 0 ? s : b->lanes[k - 1];
+    rc = run_interpolate(b->grids[g], Q + 3 * offsets[g], offsets[g + 1] - offsets[g], &ob, vals[g], vecs[g], nullptr,
+                         st, nullptr, 0);
+    if (rc != BRILLE_OK) {
+      rc = fail(rc, "grid " + std::to_string(g) + ": " + std::string(g_last_error_text()));
+      break;
+    }
+  }
+  // the caller's stream waits for every lane it forked, also after an error
+  for (int k = 0; k < n_lanes; ++k) {
+    BATCH_TRY(hipEventRecord(b->join[k], b->lanes[k]));
+    BATCH_TRY(hipStreamWaitEvent(s, b->join[k], 0));
+  }
+  return rc;
 }
 
 }  // namespace brille_amd
@@ -506,7 +549,7 @@ int brille_batch_autotune(brille_batch* b, const double* Q, const uint64_t* offs
         }
   // then the share of the work the binned pipeline takes, with that schedule
   const Schedule persistent = best;
-  for (uint64_t bmin : {1ull << 20, 1ull << 18, 1ull << 16, 1ull << 14}) {
+  for (uint64_t bmin : {1ull << 20, 1ull << 18, 1ull << 16, 1ull << 14, 1ull << 12}) {
     if (rc != BRILLE_OK) break;
     Schedule sc = persistent;
     sc.binned_min = bmin;
@@ -557,6 +600,11 @@ int brille_batch_destroy(brille_batch* b) {
   if (b->d_slots) (void)hipFree(b->d_slots);
   if (b->d_hdr) (void)hipFree(b->d_hdr);
   if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
+  for (int k = 0; k < brille_batch::kLanes; ++k) {
+    if (b->lanes[k]) (void)hipStreamDestroy(b->lanes[k]);
+    if (b->join[k]) (void)hipEventDestroy(b->join[k]);
+  }
+  if (b->fork) (void)hipEventDestroy(b->fork);
   delete b;
   return BRILLE_OK;
 }

@@ -377,8 +377,13 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t nb) {
   return x * q + (x < r ? x : r) + i;
 }
 
+// The FP32 wide-row sums are held to 168 VGPRs (three waves per SIMD, some
+// spilled): their row loads wait on memory, and the third wave hides more of
+// it than the spills cost (config-2 batch 247 -> 217 ms; four waves, 128
+// VGPRs, spill too much: 311 ms).
 template <typename Real, int L, int SC, int SV, bool Prop>
-__global__ __launch_bounds__(TILE) void k_bin_sums(GridConst g, DataConst d, ProposalConst pc, const SortRec* srt,
+__global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(L == LAYOUT_WIDE && sizeof(Real) == 4 ? 3 : 1)))
+void k_bin_sums(GridConst g, DataConst d, ProposalConst pc, const SortRec* srt,
                                                     const uint32_t* perm, uint32_t n, void* __restrict__ vals,
                                                     void* __restrict__ vecs, brille_aux_out aux,
                                                     ErrRec* __restrict__ err, uint64_t base) {
@@ -407,6 +412,11 @@ __global__ __launch_bounds__(TILE) void k_bin_sums(GridConst g, DataConst d, Pro
   const uint32_t tile0 = xcd_tile(blockIdx.x, gridDim.x) * TILE;
   const uint32_t tid = threadIdx.x;
   const uint32_t p = tile0 + tid;
+  // wide rows: blockIdx.y picks a group of modes (gridDim.y groups; the
+  // first one reports the points' errors and aux outputs)
+  const bool lead = blockIdx.y == 0;
+  const uint32_t per_group = (d.B + gridDim.y - 1) / gridDim.y;
+  const uint32_t b_lo = blockIdx.y * per_group, b_hi = b_lo + per_group;
   const bool want_phase = d.vec_rot == BRILLE_ROT_GAMMA && d.ec[1] > 0;
   const uint32_t n_atoms = d.ec[1] / 3u;
   const bool ph_in_lds = L == LAYOUT_GAMMA || (want_phase && n_atoms <= 4);
@@ -448,7 +458,7 @@ __global__ __launch_bounds__(TILE) void k_bin_sums(GridConst g, DataConst d, Pro
         node_slots(g, P);
       }
       // failed fold-ins were reported by k_bin_fold
-      if (P.status != BRILLE_PT_OK) report_point(err, base + sr.index, P.status);
+      if (P.status != BRILLE_PT_OK && lead) report_point(err, base + sr.index, P.status);
     }
   }
   // Stable partition of the tile: points the cube factor table serves first,
@@ -475,11 +485,11 @@ __global__ __launch_bounds__(TILE) void k_bin_sums(GridConst g, DataConst d, Pro
       s_out[slot] = sr.index;
       if (P.status == BRILLE_PT_OK)
         store_record<Real, L, SC>(g, d, P, r, s_ph, s_q, slot, ph_in_lds, n_atoms, !kOpFrames);
-      write_aux(aux, sr.index, P);
+      if (lead) write_aux(aux, sr.index, P);
     }
   }
   __syncthreads();
-  if (Prop && threadIdx.x == 0 && s_tried) {
+  if (Prop && lead && threadIdx.x == 0 && s_tried) {
     atomicAdd(&pc.stats[0], (unsigned long long)s_tried);
     atomicAdd(&pc.stats[1], (unsigned long long)s_verified);
   }
@@ -488,7 +498,7 @@ __global__ __launch_bounds__(TILE) void k_bin_sums(GridConst g, DataConst d, Pro
   // fewer rows in flight keep the registers for occupancy
   constexpr int GRP = sizeof(Real) == 4 ? 2 : 1;
   tile_sums<Real, true, L, SC, SV, TILE, GRP>(g, d, s_rec, s_ph, s_q, s_out, 0, npts, ph_in_lds, vals, vecs, s_wide,
-                                              table, n_tab, kOpFrames ? s_frames : nullptr, true, wg);
+                                              table, n_tab, kOpFrames ? s_frames : nullptr, true, wg, b_lo, b_hi);
 }
 
 // The row table: thread t = (block row, mode b) phases the row of a cube's or
@@ -604,6 +614,21 @@ static SumsKernel pick_sums(const DataConst& d, bool prop) {
   }
   if (wide_rows(d)) return k_bin_sums<Real, LAYOUT_WIDE, 0, 0, false>;
   return k_bin_sums<Real, LAYOUT_GENERIC, 0, 0, false>;
+}
+
+// Mode groups of the sums of a pass of m points (blockIdx.y): wide rows put
+// a tile's modes in sequence on its four waves, so a pass of few tiles would
+// leave most CUs idle; its modes are split over up to B / 4 blocks per tile
+// until the pass has about eight blocks per CU (each group repeats phase 1,
+// a few FP64 predicates per point).  Other kernels pick_sums selects (the
+// proposal's among them): one group.
+static uint32_t sum_groups(const brille_grid* G, uint32_t m, bool prop) {
+  if (prop || !wide_rows(G->d)) return 1;
+  const uint32_t tiles = (m + TILE - 1) / TILE;
+  const uint32_t want = 8u * std::max<uint32_t>(G->n_cu, 1);
+  const uint32_t max_groups = std::max<uint32_t>((G->d.B + 3) / 4, 1);
+  const uint32_t g = (want + tiles - 1) / std::max<uint32_t>(tiles, 1);
+  return std::max<uint32_t>(1, std::min(g, max_groups));
 }
 
 // SC of the one-pass sums kernel pick_sums selects, or 0 for the general ones
@@ -968,7 +993,8 @@ int run_binned_passes(brille_grid* G, const LaunchArgs& a, int precision, hipStr
     }
     // sums of pass k on the call stream
     mark(k, 3, s);
-    hipLaunchKernelGGL(sums, dim3((m + TILE - 1) / TILE), dim3(TILE), 0, s, G->g, G->d, a.prop, b.srt, b.perm, m,
+    hipLaunchKernelGGL(sums, dim3((m + TILE - 1) / TILE, sum_groups(G, m, prop)), dim3(TILE), 0, s, G->g, G->d, a.prop,
+                       b.srt, b.perm, m,
                        static_cast<char*>(a.vals) + c0 * vb, static_cast<char*>(a.vecs) + c0 * cb,
                        offset_aux(a.aux, c0), a.err, a.base + c0);
     mark(k, 4, s);

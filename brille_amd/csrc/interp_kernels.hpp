@@ -851,10 +851,9 @@ enum Layout : int {
 constexpr uint32_t kWideMax = 128;  // complex elements per row the wide phase 2 handles (two per lane)
 constexpr uint32_t kWideAtoms = kWideMax / 3;  // atoms of the longest wide Gamma row
 // LDS of the node-ordered wide sums' Gamma table: per-point phases of a chunk
-// of the tile's points, every atom (FP32: 4608 complex, 192 points of 24
-// atoms).  Those sums hold over 128 VGPRs, so two blocks per CU is their
-// occupancy with or without it.
-constexpr uint32_t kWidePhaseBytes = 36864;
+// of the tile's points, every atom (FP32: 2048 complex, 85 points of 24
+// atoms), small enough for three blocks of those sums per CU.
+constexpr uint32_t kWidePhaseBytes = 16384;
 
 __device__ __forceinline__ float wave_sum_f(float v) {
 #pragma unroll
@@ -1000,11 +999,31 @@ __device__ __forceinline__ void wide_load(const DataConst& d, const PointRec<Rea
   }
 }
 
-// The sums of mode b of a point from its loaded rows, the Gamma rotation
-// through the wave's LDS slice `lds` (it mixes the three elements of an
-// atom), and the stores.  `wc` keeps the point's rotation data while
-// `fresh` is false (the previous unit was the same point's).
+// The rows of a WideRows multiplied by their antiphase factors in place
+// (phase_elem, as the sums would per point): a table block's rows that
+// several points reuse are antiphased once.
 template <typename Real, bool TWO>
+__device__ __forceinline__ void wide_phase_rows(WideRows<Real, TWO>& w) {
+#pragma unroll
+  for (int x = 0; x < 8; ++x) {
+    Real px, py;
+    phase_elem<Real>(w.e[x], w.v0[x].x, w.v0[x].y, &px, &py);
+    w.v0[x].x = px;
+    w.v0[x].y = py;
+    if constexpr (TWO) {
+      phase_elem<Real>(w.e[x], w.v1[x].x, w.v1[x].y, &px, &py);
+      w.v1[x].x = px;
+      w.v1[x].y = py;
+    }
+  }
+}
+
+// The sums of mode b of a point from its loaded rows (PHASED: rows already
+// antiphased by wide_phase_rows), the Gamma rotation through the wave's LDS
+// slice `lds` (it mixes the three elements of an atom), and the stores.
+// `wc` keeps the point's rotation data while `fresh` is false (the previous
+// unit was the same point's).
+template <typename Real, bool TWO, bool PHASED = false>
 __device__ __forceinline__ void wide_sum_store(const GridConst& g, const DataConst& d, const PointRec<Real>& r,
                                                const double* q, uint32_t b, const WideRows<Real, TWO>& w,
                                                Real* ov_point, typename Cplx<Real>::T* oc_point,
@@ -1021,12 +1040,14 @@ __device__ __forceinline__ void wide_sum_store(const GridConst& g, const DataCon
 #pragma unroll
   for (int x = 0; x < 8; ++x) {
     const Real wx = r.w[x];
-    Real px, py;
-    phase_elem<Real>(w.e[x], w.v0[x].x, w.v0[x].y, &px, &py);
+    Real px = w.v0[x].x, py = w.v0[x].y;
+    if constexpr (!PHASED) phase_elem<Real>(w.e[x], w.v0[x].x, w.v0[x].y, &px, &py);
     a0.x = fma_r(wx, px, a0.x);
     a0.y = fma_r(wx, py, a0.y);
     if constexpr (TWO) {
-      phase_elem<Real>(w.e[x], w.v1[x].x, w.v1[x].y, &px, &py);
+      px = w.v1[x].x;
+      py = w.v1[x].y;
+      if constexpr (!PHASED) phase_elem<Real>(w.e[x], w.v1[x].x, w.v1[x].y, &px, &py);
       a1.x = fma_r(wx, px, a1.x);
       a1.y = fma_r(wx, py, a1.y);
     }
@@ -1340,7 +1361,7 @@ __device__ __forceinline__ void tile_sums(const GridConst& g, const DataConst& d
                                           void* vals_, void* vecs_, typename Cplx<Real>::T* s_wide = nullptr,
                                           bool table = false, uint32_t n_coop = 0,
                                           const GammaFrame<Real>* op_frames = nullptr, bool node_order = false,
-                                          WideGammaLds<Real> wg = {}) {
+                                          WideGammaLds<Real> wg = {}, uint32_t b_lo = 0, uint32_t b_hi = ~0u) {
   using C = typename Cplx<Real>::T;
   constexpr int NPH = TileShape<L, SC>::NPH;
   const uint32_t B = d.B;
@@ -1386,6 +1407,7 @@ __device__ __forceinline__ void tile_sums(const GridConst& g, const DataConst& d
     // With `wg`, the points go in chunks whose Gamma phases the block first
     // computes into LDS (once per point and atom instead of once per mode),
     // and the rotation rows and atom map come from per-operation LDS tables.
+    // The block sums modes b_lo .. b_hi - 1 only.
     const bool pre = gamma && wg.ph != nullptr;
     const uint32_t na = d.sc / 3u;
     const uint32_t chunk = pre ? (wg.cap / na < npts ? wg.cap / na : npts) : npts;
@@ -1403,7 +1425,8 @@ __device__ __forceinline__ void tile_sums(const GridConst& g, const DataConst& d
           }
           __syncthreads();
         }
-        for (uint32_t b = wave; b < B; b += TB / 64) {
+        const uint32_t b_end = b_hi < B ? b_hi : B;
+        for (uint32_t b = b_lo + wave; b < b_end; b += TB / 64) {
           uint32_t cached = kNoCube;
           for (uint32_t pl = c0; pl < c1; ++pl) {
             const PointRec<Real>& r = s_rec[pl];
@@ -1418,6 +1441,7 @@ __device__ __forceinline__ void tile_sums(const GridConst& g, const DataConst& d
             const bool reuse = tab && r.trow != kNoCube && r.trow == cached;
             if (!reuse) {
               wide_load<Real, TWO>(d, r, b, tab, w);
+              wide_phase_rows<Real, TWO>(w);
               cached = (tab && r.trow != kNoCube) ? r.trow : kNoCube;
             }
             if (pre) {
@@ -1433,7 +1457,7 @@ __device__ __forceinline__ void tile_sums(const GridConst& g, const DataConst& d
                 }
               }
             }
-            wide_sum_store<Real, TWO>(g, d, r, s_q + 3 * pl, b, w, ov, oc, lds, gamma, cart, wc, !pre);
+            wide_sum_store<Real, TWO, true>(g, d, r, s_q + 3 * pl, b, w, ov, oc, lds, gamma, cart, wc, !pre);
           }
         }
       }

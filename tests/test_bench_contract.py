@@ -28,7 +28,7 @@ def test_more_than_one_gpu_is_refused():
 def test_traffic_comes_from_the_committed_summary():
     bench = _bench_module()
     alg = 24 + 6 * 4 + 6 * 6 * 8  # config 4 FP32: Q in, values and vectors out
-    for path in ("learned+binned fp32", "binned fp32", "tile fp32"):
+    for path in ("learned+binned fp32", "binned fp32"):
         v = bench.measured_traffic(4, path)
         assert v is not None and v > alg, (path, v)
     assert bench.measured_traffic(4, "no such path") is None

@@ -67,14 +67,24 @@ def test_binned_fp64_matches_oracle():
     np.testing.assert_allclose(cb[:m], oc, rtol=1e-12, atol=1e-12)
 
 
-def test_binned_errors_match_tile():
-    """The first failing point (lowest index) and the message are the same on both paths."""
-    say("binned error reporting")
-    g = S.grid(4, volume_fraction=1e-4)
-    dev = capi.Grid(g.tables())
-    Q = S.q_points(4, 300_000, stream=13)
-    Q[123_457] = [np.nan, 0.0, 0.0]
-    Q[200_001] = [np.inf, 1.0, 0.0]
+@pytest.mark.parametrize("payload", ["config4", "wide"])
+def test_binned_errors_match_tile(payload):
+    """The first failing point (lowest index) and the message are the same on
+    both paths; for wide rows the binned sums split each tile's modes over
+    several blocks, of which only the first reports."""
+    say(f"binned error reporting, {payload}")
+    if payload == "config4":
+        g = S.grid(4, volume_fraction=1e-4)
+        dev = capi.Grid(g.tables())
+        Q = S.q_points(4, 300_000, stream=13)
+        bad = (123_457, 200_001)
+    else:
+        grids, Q, _ = S.batch_grids(n_grids=6, v_range=(800, 2000), n_q=20_000)
+        sc = [int(sum(g.tables()["vectors_elements"])) for g in grids]
+        dev = capi.Grid(grids[int(np.argmax(sc))].tables())
+        bad = (12_345, 17_001)
+    Q[bad[0]] = [np.nan, 0.0, 0.0]
+    Q[bad[1]] = [np.inf, 1.0, 0.0]
     lib = capi.lib()
     msgs = []
     for kernel in (capi.KERNEL_BINNED, capi.KERNEL_TILE):
@@ -83,7 +93,7 @@ def test_binned_errors_match_tile():
         lib.brille_last_error(buf, 512)
         msgs.append((rc, buf.value.decode()))
         # failed points give zero rows; the others are filled
-        assert np.all(v[123_457] == 0) and np.all(c[200_001] == 0)
+        assert np.all(v[bad[0]] == 0) and np.all(c[bad[1]] == 0)
         assert np.all(np.abs(v[:1000]).sum(axis=(1, 2)) > 0)
     assert msgs[0] == msgs[1]
     assert msgs[0][0] != 0
