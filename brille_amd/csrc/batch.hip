@@ -549,7 +549,7 @@ int brille_batch_autotune(brille_batch* b, const double* Q, const uint64_t* offs
         }
   // then the share of the work the binned pipeline takes, with that schedule
   const Schedule persistent = best;
-  for (uint64_t bmin : {1ull << 20, 1ull << 18, 1ull << 16, 1ull << 14, 1ull << 12}) {
+  for (uint64_t bmin : {1ull << 20, 1ull << 18, 1ull << 16, 1ull << 14, 1ull << 12, 1ull}) {
     if (rc != BRILLE_OK) break;
     Schedule sc = persistent;
     sc.binned_min = bmin;

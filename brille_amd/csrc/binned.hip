@@ -377,13 +377,11 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t nb) {
   return x * q + (x < r ? x : r) + i;
 }
 
-// The FP32 wide-row sums are held to 168 VGPRs (three waves per SIMD, some
-// spilled): their row loads wait on memory, and the third wave hides more of
-// it than the spills cost (config-2 batch 247 -> 217 ms; four waves, 128
-// VGPRs, spill too much: 311 ms).
+// (The FP32 wide-row sums run at two waves per SIMD, over 200 VGPRs: held
+// to three waves (168 VGPRs, some spilled) the wave-per-unit walk gained
+// 12 %, the atom-major walk lost 8 %; four waves spill too much.)
 template <typename Real, int L, int SC, int SV, bool Prop>
-__global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(L == LAYOUT_WIDE && sizeof(Real) == 4 ? 3 : 1)))
-void k_bin_sums(GridConst g, DataConst d, ProposalConst pc, const SortRec* srt,
+__global__ __launch_bounds__(TILE) void k_bin_sums(GridConst g, DataConst d, ProposalConst pc, const SortRec* srt,
                                                     const uint32_t* perm, uint32_t n, void* __restrict__ vals,
                                                     void* __restrict__ vecs, brille_aux_out aux,
                                                     ErrRec* __restrict__ err, uint64_t base) {

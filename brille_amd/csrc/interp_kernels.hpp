@@ -851,9 +851,10 @@ enum Layout : int {
 constexpr uint32_t kWideMax = 128;  // complex elements per row the wide phase 2 handles (two per lane)
 constexpr uint32_t kWideAtoms = kWideMax / 3;  // atoms of the longest wide Gamma row
 // LDS of the node-ordered wide sums' Gamma table: per-point phases of a chunk
-// of the tile's points, every atom (FP32: 2048 complex, 85 points of 24
-// atoms), small enough for three blocks of those sums per CU.
-constexpr uint32_t kWidePhaseBytes = 16384;
+// of the tile's points, every atom (FP32: 4608 complex, 192 points of 24
+// atoms).  Those sums hold over 200 VGPRs, so two blocks per CU is their
+// occupancy with or without it.
+constexpr uint32_t kWidePhaseBytes = 36864;
 
 __device__ __forceinline__ float wave_sum_f(float v) {
 #pragma unroll
@@ -1411,59 +1412,147 @@ __device__ __forceinline__ void tile_sums(const GridConst& g, const DataConst& d
     const bool pre = gamma && wg.ph != nullptr;
     const uint32_t na = d.sc / 3u;
     const uint32_t chunk = pre ? (wg.cap / na < npts ? wg.cap / na : npts) : npts;
-    auto walk = [&](auto two_tag) {
-      constexpr bool TWO = decltype(two_tag)::value;
-      WideRows<Real, TWO> w;
-      for (uint32_t c0 = 0; c0 < npts; c0 += chunk) {
-        const uint32_t c1 = c0 + chunk < npts ? c0 + chunk : npts;
-        if (pre) {
-          __syncthreads();  // the previous chunk's phases are read
-          for (uint32_t k = threadIdx.x; k < (c1 - c0) * na; k += TB) {
-            const uint32_t pl = c0 + k / na, a = k - (k / na) * na;
-            const PointRec<Real>& r = s_rec[pl];
-            wg.ph[k] = r.nslot ? gamma_phase<Real>(d, s_q + 3 * pl, r.invrot, a) : Elem<C>::zero();
+    const uint32_t b_end = b_hi < B ? b_hi : B;
+    // Atom-major (Gamma rows with the factor table and at most one value per
+    // atom): lane l takes atom a = l % na of mode l / na, so a wave sums
+    // 64 / na modes of a point at once, each lane its atom's three elements:
+    // the rotation needs no exchange between lanes.  Same per-element
+    // arithmetic as the wave-per-unit walk below, so the same bits.  Points
+    // without a table block are left to that walk.
+    // (Staging a wave's rotated rows in LDS, to write them as one contiguous
+    // run per point instead of three 8-byte stores per lane at a permuted
+    // atom, saved no time: config-2 batch 140 -> 145 ms.)
+    const bool by_atom = pre && tab && d.sv <= na;
+    auto atom_walk = [&](uint32_t c0, uint32_t c1) {
+      const uint32_t mpw = 64u / na;  // modes per wave pass
+      const uint32_t m = lane / na, a = lane - m * na;
+      const uint32_t nmodes = b_end > b_lo ? b_end - b_lo : 0u;
+      const uint32_t passes = (nmodes + mpw - 1) / mpw;
+      const C* fac = wide_factors<Real>(d);
+      const uint32_t sc = d.sc, sv = d.sv;
+      for (uint32_t t = wave; t < passes; t += TB / 64) {
+        const uint32_t b = b_lo + t * mpw + m;
+        const bool on = m < mpw && b < b_end;
+        C v[8][3];
+        Real vv[8];
+        uint32_t cached = kNoCube;
+        for (uint32_t pl = c0; pl < c1; ++pl) {
+          const PointRec<Real>& r = s_rec[pl];
+          const uint64_t row = ((out_row ? (uint64_t)out_row[pl] : row0 + pl)) * B;
+          Real* ov = vals + row * sv + (uint64_t)b * sv;
+          C* oc = vecs + row * sc + (uint64_t)b * sc;
+          if (r.nslot == 0) {
+            if (on) {
+#pragma unroll
+              for (int i = 0; i < 3; ++i) oc[3u * a + i] = Elem<C>::zero();
+              if (a < sv) ov[a] = 0;
+            }
+            continue;
           }
-          __syncthreads();
-        }
-        const uint32_t b_end = b_hi < B ? b_hi : B;
-        for (uint32_t b = b_lo + wave; b < b_end; b += TB / 64) {
-          uint32_t cached = kNoCube;
-          for (uint32_t pl = c0; pl < c1; ++pl) {
-            const PointRec<Real>& r = s_rec[pl];
-            const uint64_t row = ((out_row ? (uint64_t)out_row[pl] : row0 + pl)) * B;
-            Real* ov = vals + row * d.sv;
-            C* oc = vecs + row * d.sc;
-            if (r.nslot == 0) {
-              for (uint32_t e = lane; e < d.sv; e += 64) ov[(uint64_t)b * d.sv + e] = 0;
-              for (uint32_t e = lane; e < d.sc; e += 64) oc[(uint64_t)b * d.sc + e] = Elem<C>::zero();
-              continue;
-            }
-            const bool reuse = tab && r.trow != kNoCube && r.trow == cached;
-            if (!reuse) {
-              wide_load<Real, TWO>(d, r, b, tab, w);
-              wide_phase_rows<Real, TWO>(w);
-              cached = (tab && r.trow != kNoCube) ? r.trow : kNoCube;
-            }
-            if (pre) {
+          if (r.trow == kNoCube) continue;
+          if (r.trow != cached) {
+            cached = r.trow;
 #pragma unroll
-              for (int h = 0; h < (TWO ? 2 : 1); ++h) {
-                const uint32_t el = lane + 64u * h;
-                if (el < d.sc) {
-                  const uint32_t a = el / 3u, i = el - 3u * a;
-                  wc.ph[h] = wg.ph[(pl - c0) * na + a];
+            for (int x = 0; x < 8; ++x) {
+              if (on) {
+                const Real* rw = row_at<Real>(d, r.idx[x], b);
+                const C* rc = reinterpret_cast<const C*>(rw) + 3u * a;
+                const C e = x < (int)r.nslot ? fac[r.trow + (uint32_t)x * B + b] : C{(Real)1, (Real)0};
 #pragma unroll
-                  for (int j = 0; j < 3; ++j) wc.R[h][j] = wg.rot[9u * r.invrot + 3u * i + j];
-                  wc.dst[h] = 3u * wg.dst[r.invrot * na + a] + i;
+                for (int i = 0; i < 3; ++i) {
+                  const C el = rc[i];
+                  phase_elem<Real>(e, el.x, el.y, &v[x][i].x, &v[x][i].y);
                 }
+                vv[x] = a < sv ? rw[2 * sc + a] : (Real)0;
               }
             }
-            wide_sum_store<Real, TWO, true>(g, d, r, s_q + 3 * pl, b, w, ov, oc, lds, gamma, cart, wc, !pre);
+          }
+          if (!on) continue;
+          C acc[3] = {Elem<C>::zero(), Elem<C>::zero(), Elem<C>::zero()};
+          Real va = 0;
+#pragma unroll
+          for (int x = 0; x < 8; ++x) {
+            const Real wx = r.w[x];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+              acc[i].x = fma_r(wx, v[x][i].x, acc[i].x);
+              acc[i].y = fma_r(wx, v[x][i].y, acc[i].y);
+            }
+            va = fma_r(wx, vv[x], va);
+          }
+          if (a < sv) __builtin_nontemporal_store(va, ov + a);
+          // rip_gamma_complex  src/interpolator_gamma.tpp:100-114, atom by atom
+          const Real* R = wg.rot + 9u * r.invrot;
+          const C ph = wg.ph[(pl - c0) * na + a];
+          C* dst = oc + 3u * wg.dst[r.invrot * na + a];
+          typedef Real real2 __attribute__((ext_vector_type(2)));
+#pragma unroll
+          for (int i = 0; i < 3; ++i) {
+            C tt = Elem<C>::zero();
+#pragma unroll
+            for (int j = 0; j < 3; ++j) tt = Elem<C>::add(tt, Elem<C>::scale(acc[j], R[3 * i + j]));
+            const C o = cmul(ph, tt);
+            __builtin_nontemporal_store(real2{o.x, o.y}, reinterpret_cast<real2*>(dst + i));
           }
         }
       }
     };
-    if (two) walk(std::true_type{});
-    else walk(std::false_type{});
+    // Wave per (point, mode): wave w takes modes w, w + TB/64, ... of the
+    // block's group; with by_atom only the points the atom walk leaves.
+    auto walk = [&](auto two_tag, uint32_t c0, uint32_t c1, bool untabled_only) {
+      constexpr bool TWO = decltype(two_tag)::value;
+      WideRows<Real, TWO> w;
+      for (uint32_t b = b_lo + wave; b < b_end; b += TB / 64) {
+        uint32_t cached = kNoCube;
+        for (uint32_t pl = c0; pl < c1; ++pl) {
+          const PointRec<Real>& r = s_rec[pl];
+          if (untabled_only && (r.nslot == 0 || r.trow != kNoCube)) continue;
+          const uint64_t row = ((out_row ? (uint64_t)out_row[pl] : row0 + pl)) * B;
+          Real* ov = vals + row * d.sv;
+          C* oc = vecs + row * d.sc;
+          if (r.nslot == 0) {
+            for (uint32_t e = lane; e < d.sv; e += 64) ov[(uint64_t)b * d.sv + e] = 0;
+            for (uint32_t e = lane; e < d.sc; e += 64) oc[(uint64_t)b * d.sc + e] = Elem<C>::zero();
+            continue;
+          }
+          const bool reuse = tab && r.trow != kNoCube && r.trow == cached;
+          if (!reuse) {
+            wide_load<Real, TWO>(d, r, b, tab, w);
+            wide_phase_rows<Real, TWO>(w);
+            cached = (tab && r.trow != kNoCube) ? r.trow : kNoCube;
+          }
+          if (pre) {
+#pragma unroll
+            for (int h = 0; h < (TWO ? 2 : 1); ++h) {
+              const uint32_t el = lane + 64u * h;
+              if (el < d.sc) {
+                const uint32_t a = el / 3u, i = el - 3u * a;
+                wc.ph[h] = wg.ph[(pl - c0) * na + a];
+#pragma unroll
+                for (int j = 0; j < 3; ++j) wc.R[h][j] = wg.rot[9u * r.invrot + 3u * i + j];
+                wc.dst[h] = 3u * wg.dst[r.invrot * na + a] + i;
+              }
+            }
+          }
+          wide_sum_store<Real, TWO, true>(g, d, r, s_q + 3 * pl, b, w, ov, oc, lds, gamma, cart, wc, !pre);
+        }
+      }
+    };
+    for (uint32_t c0 = 0; c0 < npts; c0 += chunk) {
+      const uint32_t c1 = c0 + chunk < npts ? c0 + chunk : npts;
+      if (pre) {
+        __syncthreads();  // the previous chunk's phases are read
+        for (uint32_t k = threadIdx.x; k < (c1 - c0) * na; k += TB) {
+          const uint32_t pl = c0 + k / na, a = k - (k / na) * na;
+          const PointRec<Real>& r = s_rec[pl];
+          wg.ph[k] = r.nslot ? gamma_phase<Real>(d, s_q + 3 * pl, r.invrot, a) : Elem<C>::zero();
+        }
+        __syncthreads();
+      }
+      if (by_atom) atom_walk(c0, c1);
+      if (two) walk(std::true_type{}, c0, c1, by_atom);
+      else walk(std::false_type{}, c0, c1, by_atom);
+    }
     return;
   }
   // Gamma frames: per point in s_q, or per point-group operation in op_frames
