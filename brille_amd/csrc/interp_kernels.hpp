@@ -1430,34 +1430,31 @@ __device__ __forceinline__ void tile_sums(const GridConst& g, const DataConst& d
       const uint32_t passes = (nmodes + mpw - 1) / mpw;
       const C* fac = wide_factors<Real>(d);
       const uint32_t sc = d.sc, sv = d.sv;
+      // the points this walk takes: located, with a table block
+      auto taken = [&](uint32_t pl) { return s_rec[pl].nslot != 0 && s_rec[pl].trow != kNoCube; };
       for (uint32_t t = wave; t < passes; t += TB / 64) {
         const uint32_t b = b_lo + t * mpw + m;
         const bool on = m < mpw && b < b_end;
-        C v[8][3];
-        Real vv[8];
-        uint32_t cached = kNoCube;
-        for (uint32_t pl = c0; pl < c1; ++pl) {
-          const PointRec<Real>& r = s_rec[pl];
-          const uint64_t row = ((out_row ? (uint64_t)out_row[pl] : row0 + pl)) * B;
-          Real* ov = vals + row * sv + (uint64_t)b * sv;
-          C* oc = vecs + row * sc + (uint64_t)b * sc;
-          if (r.nslot == 0) {
-            if (on) {
-#pragma unroll
-              for (int i = 0; i < 3; ++i) oc[3u * a + i] = Elem<C>::zero();
-              if (a < sv) ov[a] = 0;
-            }
-            continue;
-          }
-          if (r.trow == kNoCube) continue;
-          if (r.trow != cached) {
-            cached = r.trow;
+        // Runs of consecutive points of one table block: the block's rows of
+        // this lane's mode and atom are loaded and antiphased once per run,
+        // and stay loop-invariant over the run's points.
+        uint32_t p0 = c0;
+        while (true) {
+          while (p0 < c1 && !taken(p0)) ++p0;
+          if (p0 >= c1) break;
+          const uint32_t trow = s_rec[p0].trow;
+          uint32_t p1 = p0 + 1;
+          while (p1 < c1 && taken(p1) && s_rec[p1].trow == trow) ++p1;
+          C v[8][3];
+          Real vv[8];
+          {
+            const PointRec<Real>& rb = s_rec[p0];
 #pragma unroll
             for (int x = 0; x < 8; ++x) {
               if (on) {
-                const Real* rw = row_at<Real>(d, r.idx[x], b);
+                const Real* rw = row_at<Real>(d, rb.idx[x], b);
                 const C* rc = reinterpret_cast<const C*>(rw) + 3u * a;
-                const C e = x < (int)r.nslot ? fac[r.trow + (uint32_t)x * B + b] : C{(Real)1, (Real)0};
+                const C e = x < (int)rb.nslot ? fac[trow + (uint32_t)x * B + b] : C{(Real)1, (Real)0};
 #pragma unroll
                 for (int i = 0; i < 3; ++i) {
                   const C el = rc[i];
@@ -1467,33 +1464,40 @@ __device__ __forceinline__ void tile_sums(const GridConst& g, const DataConst& d
               }
             }
           }
-          if (!on) continue;
-          C acc[3] = {Elem<C>::zero(), Elem<C>::zero(), Elem<C>::zero()};
-          Real va = 0;
+          for (uint32_t pl = p0; pl < p1; ++pl) {
+            if (!on) continue;
+            const PointRec<Real>& r = s_rec[pl];
+            const uint64_t row = ((out_row ? (uint64_t)out_row[pl] : row0 + pl)) * B;
+            Real* ov = vals + row * sv + (uint64_t)b * sv;
+            C* oc = vecs + row * sc + (uint64_t)b * sc;
+            C acc[3] = {Elem<C>::zero(), Elem<C>::zero(), Elem<C>::zero()};
+            Real va = 0;
 #pragma unroll
-          for (int x = 0; x < 8; ++x) {
-            const Real wx = r.w[x];
+            for (int x = 0; x < 8; ++x) {
+              const Real wx = r.w[x];
+#pragma unroll
+              for (int i = 0; i < 3; ++i) {
+                acc[i].x = fma_r(wx, v[x][i].x, acc[i].x);
+                acc[i].y = fma_r(wx, v[x][i].y, acc[i].y);
+              }
+              va = fma_r(wx, vv[x], va);
+            }
+            if (a < sv) __builtin_nontemporal_store(va, ov + a);
+            // rip_gamma_complex  src/interpolator_gamma.tpp:100-114, atom by atom
+            const Real* R = wg.rot + 9u * r.invrot;
+            const C ph = wg.ph[(pl - c0) * na + a];
+            C* dst = oc + 3u * wg.dst[r.invrot * na + a];
+            typedef Real real2 __attribute__((ext_vector_type(2)));
 #pragma unroll
             for (int i = 0; i < 3; ++i) {
-              acc[i].x = fma_r(wx, v[x][i].x, acc[i].x);
-              acc[i].y = fma_r(wx, v[x][i].y, acc[i].y);
+              C tt = Elem<C>::zero();
+#pragma unroll
+              for (int j = 0; j < 3; ++j) tt = Elem<C>::add(tt, Elem<C>::scale(acc[j], R[3 * i + j]));
+              const C o = cmul(ph, tt);
+              __builtin_nontemporal_store(real2{o.x, o.y}, reinterpret_cast<real2*>(dst + i));
             }
-            va = fma_r(wx, vv[x], va);
           }
-          if (a < sv) __builtin_nontemporal_store(va, ov + a);
-          // rip_gamma_complex  src/interpolator_gamma.tpp:100-114, atom by atom
-          const Real* R = wg.rot + 9u * r.invrot;
-          const C ph = wg.ph[(pl - c0) * na + a];
-          C* dst = oc + 3u * wg.dst[r.invrot * na + a];
-          typedef Real real2 __attribute__((ext_vector_type(2)));
-#pragma unroll
-          for (int i = 0; i < 3; ++i) {
-            C tt = Elem<C>::zero();
-#pragma unroll
-            for (int j = 0; j < 3; ++j) tt = Elem<C>::add(tt, Elem<C>::scale(acc[j], R[3 * i + j]));
-            const C o = cmul(ph, tt);
-            __builtin_nontemporal_store(real2{o.x, o.y}, reinterpret_cast<real2*>(dst + i));
-          }
+          p0 = p1;
         }
       }
     };
@@ -1506,7 +1510,7 @@ __device__ __forceinline__ void tile_sums(const GridConst& g, const DataConst& d
         uint32_t cached = kNoCube;
         for (uint32_t pl = c0; pl < c1; ++pl) {
           const PointRec<Real>& r = s_rec[pl];
-          if (untabled_only && (r.nslot == 0 || r.trow != kNoCube)) continue;
+          if (untabled_only && r.nslot != 0 && r.trow != kNoCube) continue;
           const uint64_t row = ((out_row ? (uint64_t)out_row[pl] : row0 + pl)) * B;
           Real* ov = vals + row * d.sv;
           C* oc = vecs + row * d.sc;
