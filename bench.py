@@ -54,6 +54,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline (profiling runs)")
     ap.add_argument("--no-learned", action="store_true", help="config 4: skip training the tetrahedron proposal")
     ap.add_argument("--no-batch", action="store_true", help="config 4: skip the configuration-2 batch leg")
+    ap.add_argument("--batch-schedule", type=str, default=None,
+                    help="config 2: tile,blocks_per_cu,tiles_per_grab,order,binned_min instead of autotuning "
+                         "(counter runs serialise kernels, which changes what the autotuner picks)")
     ap.add_argument("--no-sort", action="store_true", help="config 4: skip the mode-sort leg")
     ap.add_argument("--sort-only", action="store_true",
                     help="only the mode-sort leg on the config-4 grid (profiling runs)")
@@ -289,7 +292,12 @@ def batch_leg(args, capi, S, with_baselines):
     batch = capi.Batch(devs)
     dQ = capi.DeviceArray.from_host(Q)
     vals, vecs = batch.outputs(offsets, capi.FP32)
-    capi.check(batch.run(dQ, offsets, vals, vecs, precision=capi.FP32, autotune=True))
+    if getattr(args, "batch_schedule", None):
+        tp, bpc, grab, order, bmin = [int(float(x)) for x in args.batch_schedule.split(",")]
+        batch.set_schedule(tp, bpc, grab, order, binned_min=bmin)
+        capi.check(batch.run(dQ, offsets, vals, vecs, precision=capi.FP32))
+    else:
+        capi.check(batch.run(dQ, offsets, vals, vecs, precision=capi.FP32, autotune=True))
     sch = batch.schedule()
     log(f"autotuned schedule: {sch}")
     for _ in range(args.warmup):

@@ -1377,6 +1377,95 @@ __device__ __forceinline__ void tile_sums(const GridConst& g, const DataConst& d
     WideCache<Real> wc;
     const bool two = d.sc > 64;
     const bool tab = wide_factors<Real>(d) != nullptr;
+    if (!node_order && gamma && tab && d.sv <= d.sc / 3u) {
+      // Point-major, atom-major (Gamma rows with the factor table): wave w
+      // takes units w, w + TB/64, ... of (point, pass), a pass being 64 /
+      // n_atoms modes with a lane per (mode, atom) as in the node-ordered
+      // walk below; the wave keeps the point's phase, rotation rows and
+      // output atom while it stays on the point.  Points without a table
+      // block take the wave-per-unit sums for the pass's modes.
+      const uint32_t na = d.sc / 3u, sc = d.sc, sv = d.sv;
+      const uint32_t mpw = 64u / na;
+      const uint32_t m = lane / na, a = lane - m * na;
+      const uint32_t passes = (B + mpw - 1) / mpw;
+      const C* fac = wide_factors<Real>(d);
+      uint32_t last = 0xFFFFFFFFu;
+      C ph = Elem<C>::zero();
+      Real R[9];
+      uint32_t dsta = 0;
+      for (uint32_t c = wave; c < npts * passes; c += TB / 64) {
+        const uint32_t pl = c / passes, t = c - pl * passes;
+        const PointRec<Real>& r = s_rec[pl];
+        const uint64_t row = ((out_row ? (uint64_t)out_row[pl] : row0 + pl)) * B;
+        if (r.nslot == 0 || r.trow == kNoCube) {
+          Real* ovp = vals + row * sv;
+          C* ocp = vecs + row * sc;
+          const uint32_t bend = (t + 1) * mpw < B ? (t + 1) * mpw : B;
+          for (uint32_t b = t * mpw; b < bend; ++b) {
+            if (r.nslot == 0) {
+              for (uint32_t e = lane; e < sv; e += 64) ovp[(uint64_t)b * sv + e] = 0;
+              for (uint32_t e = lane; e < sc; e += 64) ocp[(uint64_t)b * sc + e] = Elem<C>::zero();
+            } else if (two) {
+              unit_wide<Real, true>(g, d, r, s_q + 3 * pl, b, ovp, ocp, lds, gamma, cart, wc, true, tab);
+            } else {
+              unit_wide<Real, false>(g, d, r, s_q + 3 * pl, b, ovp, ocp, lds, gamma, cart, wc, true, tab);
+            }
+          }
+          continue;
+        }
+        if (pl != last) {
+          last = pl;
+          if (m < mpw) {
+            ph = gamma_phase<Real>(d, s_q + 3 * pl, r.invrot, a);
+#pragma unroll
+            for (int k = 0; k < 9; ++k)
+              R[k] = cart ? (Real)d.cart_rot[9 * (uint64_t)r.invrot + k] : (Real)g.rot[9 * (uint64_t)r.invrot + k];
+            dsta = d.F0[a * d.gamma_ops + r.invrot];
+          }
+        }
+        const uint32_t b = t * mpw + m;
+        if (m >= mpw || b >= B) continue;
+        C v[8][3];
+        Real vv[8];
+#pragma unroll
+        for (int x = 0; x < 8; ++x) {
+          const Real* rw = row_at<Real>(d, r.idx[x], b);
+          const C* rc = reinterpret_cast<const C*>(rw) + 3u * a;
+          const C e = x < (int)r.nslot ? fac[r.trow + (uint32_t)x * B + b] : C{(Real)1, (Real)0};
+#pragma unroll
+          for (int i = 0; i < 3; ++i) {
+            const C el = rc[i];
+            phase_elem<Real>(e, el.x, el.y, &v[x][i].x, &v[x][i].y);
+          }
+          vv[x] = a < sv ? rw[2 * sc + a] : (Real)0;
+        }
+        C acc[3] = {Elem<C>::zero(), Elem<C>::zero(), Elem<C>::zero()};
+        Real va = 0;
+#pragma unroll
+        for (int x = 0; x < 8; ++x) {
+          const Real wx = r.w[x];
+#pragma unroll
+          for (int i = 0; i < 3; ++i) {
+            acc[i].x = fma_r(wx, v[x][i].x, acc[i].x);
+            acc[i].y = fma_r(wx, v[x][i].y, acc[i].y);
+          }
+          va = fma_r(wx, vv[x], va);
+        }
+        if (a < sv) __builtin_nontemporal_store(va, vals + row * sv + (uint64_t)b * sv + a);
+        // rip_gamma_complex  src/interpolator_gamma.tpp:100-114, atom by atom
+        C* dst = vecs + row * sc + (uint64_t)b * sc + 3u * dsta;
+        typedef Real real2 __attribute__((ext_vector_type(2)));
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          C tt = Elem<C>::zero();
+#pragma unroll
+          for (int j = 0; j < 3; ++j) tt = Elem<C>::add(tt, Elem<C>::scale(acc[j], R[3 * i + j]));
+          const C o = cmul(ph, tt);
+          __builtin_nontemporal_store(real2{o.x, o.y}, reinterpret_cast<real2*>(dst + i));
+        }
+      }
+      return;
+    }
     if (!node_order) {
       // Point-major: wave w takes units w, w + TB/64, ... of (point, mode),
       // so it stays on a point for its share of the modes and keeps the
