@@ -384,13 +384,15 @@ template <typename Real, int L, int SC, int SV, bool Prop>
 __global__ __launch_bounds__(TILE) void k_bin_sums(GridConst g, DataConst d, ProposalConst pc, const SortRec* srt,
                                                     const uint32_t* perm, uint32_t n, void* __restrict__ vals,
                                                     void* __restrict__ vecs, brille_aux_out aux,
-                                                    ErrRec* __restrict__ err, uint64_t base) {
+                                                    ErrRec* __restrict__ err, uint64_t base, uint32_t* rest,
+                                                    uint32_t* rest_n) {
   using C = typename Cplx<Real>::T;
   using S = TileShape<L, SC, TILE, Real>;
   __shared__ PointRec<Real> s_rec[TILE];
   // wide rows: the phase table of the node-ordered Gamma sums instead
   constexpr bool kWide = L == LAYOUT_WIDE;
-  constexpr uint32_t kPh = kWide ? kWidePhaseBytes / sizeof(C) : TILE * S::NPH;
+  constexpr uint32_t kPh =
+      kWide ? (SC == kWideAtom ? kWideAtomPhaseBytes : kWidePhaseBytes) / sizeof(C) : TILE * S::NPH;
   __shared__ C s_ph[kPh];
   __shared__ Real s_wrot[kWide ? 9 * kMaxOps : 1];
   __shared__ uint8_t s_wdst[kWide ? kWideAtoms * kMaxOps : 1];
@@ -407,7 +409,14 @@ __global__ __launch_bounds__(TILE) void k_bin_sums(GridConst g, DataConst d, Pro
     s_tried = 0;
     s_verified = 0;
   }
+  // wide rows, kWideAtom: the located points without a table block are
+  // listed in rest[0 .. *rest_n) (record indices) for the kWideRest launch,
+  // which takes its points from that list
+  constexpr bool kAtom = L == LAYOUT_WIDE && SC == kWideAtom;
+  constexpr bool kRest = L == LAYOUT_WIDE && SC == kWideRest;
+  if constexpr (kRest) n = min(n, *rest_n);
   const uint32_t tile0 = xcd_tile(blockIdx.x, gridDim.x) * TILE;
+  if (kRest && tile0 >= n) return;  // the grid covers the largest possible list
   const uint32_t tid = threadIdx.x;
   const uint32_t p = tile0 + tid;
   // wide rows: blockIdx.y picks a group of modes (gridDim.y groups; the
@@ -433,8 +442,10 @@ __global__ __launch_bounds__(TILE) void k_bin_sums(GridConst g, DataConst d, Pro
   Located P;
   P.status = BRILLE_PT_OK;
   P.cube = kNoCube;
+  uint32_t ridx = 0;
   if (live) {
-    sr = srt[perm[p] & ~kPermPoly];
+    ridx = kRest ? rest[p] : (perm[p] & ~kPermPoly);
+    sr = srt[ridx];
     P.q[0] = sr.q[0]; P.q[1] = sr.q[1]; P.q[2] = sr.q[2];
     P.tau[0] = P.tau[1] = P.tau[2] = 0;
     P.node = sr.node;
@@ -457,6 +468,7 @@ __global__ __launch_bounds__(TILE) void k_bin_sums(GridConst g, DataConst d, Pro
       }
       // failed fold-ins were reported by k_bin_fold
       if (P.status != BRILLE_PT_OK && lead) report_point(err, base + sr.index, P.status);
+      if (kAtom && lead && P.status == BRILLE_PT_OK && table_row(d, P) == kNoCube) rest[atomicAdd(rest_n, 1u)] = ridx;
     }
   }
   // Stable partition of the tile: points the cube factor table serves first,
@@ -483,7 +495,7 @@ __global__ __launch_bounds__(TILE) void k_bin_sums(GridConst g, DataConst d, Pro
       s_out[slot] = sr.index;
       if (P.status == BRILLE_PT_OK)
         store_record<Real, L, SC>(g, d, P, r, s_ph, s_q, slot, ph_in_lds, n_atoms, !kOpFrames);
-      if (lead) write_aux(aux, sr.index, P);
+      if (lead && !kRest) write_aux(aux, sr.index, P);
     }
   }
   __syncthreads();
@@ -582,7 +594,15 @@ __global__ __launch_bounds__(256) void k_wide_factors(GridConst g, DataConst d, 
 }
 
 using SumsKernel = void (*)(GridConst, DataConst, ProposalConst, const SortRec*, const uint32_t*, uint32_t, void*,
-                           void*, brille_aux_out, ErrRec*, uint64_t);
+                           void*, brille_aux_out, ErrRec*, uint64_t, uint32_t*, uint32_t*);
+
+// Gamma rows too long for one lane that the atom-major walk takes whole: the
+// derived antiphase factor table exists and there is at most one value per atom
+static bool wide_by_atom(const DataConst& d, int precision) {
+  const bool fac = precision == BRILLE_FP32 ? d.wide_fac32 != nullptr : d.wide_fac64 != nullptr;
+  return payload_layout(d) == LAYOUT_GAMMA && d.vec_rot == BRILLE_ROT_GAMMA && d.ec[1] > 0 && fac &&
+         d.sv <= d.sc / 3u;
+}
 
 template <typename Real>
 static SumsKernel pick_sums(const DataConst& d, bool prop) {
@@ -610,7 +630,9 @@ static SumsKernel pick_sums(const DataConst& d, bool prop) {
       default: break;
     }
   }
-  if (wide_rows(d)) return k_bin_sums<Real, LAYOUT_WIDE, 0, 0, false>;
+  if (wide_rows(d))
+    return wide_by_atom(d, sizeof(Real) == 4 ? BRILLE_FP32 : BRILLE_FP64) ? k_bin_sums<Real, LAYOUT_WIDE, kWideAtom, 0, false>
+                                                                         : k_bin_sums<Real, LAYOUT_WIDE, kWideAll, 0, false>;
   return k_bin_sums<Real, LAYOUT_GENERIC, 0, 0, false>;
 }
 
@@ -946,6 +968,9 @@ int run_binned_passes(brille_grid* G, const LaunchArgs& a, int precision, hipStr
     if (stage_ms) (void)hipEventRecord(tev[k * 5 + j], st);
   };
   const SumsKernel sums = precision == BRILLE_FP32 ? pick_sums<float>(G->d, prop) : pick_sums<double>(G->d, prop);
+  const bool by_atom = !prop && wide_rows(G->d) && wide_by_atom(G->d, precision);
+  const SumsKernel rest_sums = precision == BRILLE_FP32 ? k_bin_sums<float, LAYOUT_WIDE, kWideRest, 0, false>
+                                                        : k_bin_sums<double, LAYOUT_WIDE, kWideRest, 0, false>;
   const size_t rs = precision == BRILLE_FP32 ? 4 : 8;
   const size_t vb = (size_t)G->d.B * G->d.sv * rs, cb = (size_t)G->d.B * G->d.sc * 2 * rs;
   const size_t lds1 = (size_t)(nb + 1) * 4;
@@ -991,10 +1016,15 @@ int run_binned_passes(brille_grid* G, const LaunchArgs& a, int precision, hipStr
     }
     // sums of pass k on the call stream
     mark(k, 3, s);
-    hipLaunchKernelGGL(sums, dim3((m + TILE - 1) / TILE, sum_groups(G, m, prop)), dim3(TILE), 0, s, G->g, G->d, a.prop,
-                       b.srt, b.perm, m,
+    const dim3 sgrid((m + TILE - 1) / TILE, sum_groups(G, m, prop));
+    if (by_atom) zero_words(b.pcount, 1, s);
+    hipLaunchKernelGGL(sums, sgrid, dim3(TILE), 0, s, G->g, G->d, a.prop, b.srt, b.perm, m,
                        static_cast<char*>(a.vals) + c0 * vb, static_cast<char*>(a.vecs) + c0 * cb,
-                       offset_aux(a.aux, c0), a.err, a.base + c0);
+                       offset_aux(a.aux, c0), a.err, a.base + c0, b.plist, b.pcount);
+    if (by_atom)  // the listed points without a table block
+      hipLaunchKernelGGL(rest_sums, sgrid, dim3(TILE), 0, s, G->g, G->d, a.prop, b.srt, b.perm, m,
+                         static_cast<char*>(a.vals) + c0 * vb, static_cast<char*>(a.vecs) + c0 * cb,
+                         offset_aux(a.aux, c0), a.err, a.base + c0, b.plist, b.pcount);
     mark(k, 4, s);
     if (halves > 1) (void)hipEventRecord(G->bin_ev[EV_DONE + h], s);
   }

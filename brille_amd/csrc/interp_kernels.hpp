@@ -850,11 +850,20 @@ enum Layout : int {
 };
 constexpr uint32_t kWideMax = 128;  // complex elements per row the wide phase 2 handles (two per lane)
 constexpr uint32_t kWideAtoms = kWideMax / 3;  // atoms of the longest wide Gamma row
+// SC of the node-ordered wide sums (LAYOUT_WIDE): every point (kWideAll);
+// only the atom-major walk, the points without a table block listed for a
+// second launch (kWideAtom); that second launch over the listed points
+// (kWideRest)
+constexpr int kWideAll = 0, kWideAtom = 1, kWideRest = 2;
 // LDS of the node-ordered wide sums' Gamma table: per-point phases of a chunk
 // of the tile's points, every atom (FP32: 4608 complex, 192 points of 24
 // atoms).  Those sums hold over 200 VGPRs, so two blocks per CU is their
 // occupancy with or without it.
 constexpr uint32_t kWidePhaseBytes = 36864;
+// the same for the atom-only sums (kWideAtom below), whose 109 VGPRs allow
+// four waves per SIMD when a block's LDS stays under 40 KB: 8 KB, 42 points
+// of 24 atoms per chunk (config-2 batch 85 -> 76 ms)
+constexpr uint32_t kWideAtomPhaseBytes = 8192;
 
 __device__ __forceinline__ float wave_sum_f(float v) {
 #pragma unroll
@@ -1511,7 +1520,9 @@ __device__ __forceinline__ void tile_sums(const GridConst& g, const DataConst& d
     // (Staging a wave's rotated rows in LDS, to write them as one contiguous
     // run per point instead of three 8-byte stores per lane at a permuted
     // atom, saved no time: config-2 batch 140 -> 145 ms.)
-    const bool by_atom = pre && tab && d.sv <= na;
+    // kWideAtom: the caller chose it for such rows; kWideRest: no point here
+    // has a table block
+    const bool by_atom = SC == kWideAtom || (SC == kWideAll && pre && tab && d.sv <= na);
     auto atom_walk = [&](uint32_t c0, uint32_t c1) {
       const uint32_t mpw = 64u / na;  // modes per wave pass
       const uint32_t m = lane / na, a = lane - m * na;
@@ -1529,7 +1540,17 @@ __device__ __forceinline__ void tile_sums(const GridConst& g, const DataConst& d
         // and stay loop-invariant over the run's points.
         uint32_t p0 = c0;
         while (true) {
-          while (p0 < c1 && !taken(p0)) ++p0;
+          while (p0 < c1 && !taken(p0)) {
+            // without the wave-per-unit walk the unlocated points are zeroed here
+            if (SC == kWideAtom && s_rec[p0].nslot == 0 && on) {
+              const uint64_t row = ((out_row ? (uint64_t)out_row[p0] : row0 + p0)) * B;
+              C* oc = vecs + row * sc + (uint64_t)b * sc;
+#pragma unroll
+              for (int i = 0; i < 3; ++i) oc[3u * a + i] = Elem<C>::zero();
+              if (a < sv) vals[row * sv + (uint64_t)b * sv + a] = 0;
+            }
+            ++p0;
+          }
           if (p0 >= c1) break;
           const uint32_t trow = s_rec[p0].trow;
           uint32_t p1 = p0 + 1;
@@ -1643,8 +1664,13 @@ __device__ __forceinline__ void tile_sums(const GridConst& g, const DataConst& d
         __syncthreads();
       }
       if (by_atom) atom_walk(c0, c1);
-      if (two) walk(std::true_type{}, c0, c1, by_atom);
-      else walk(std::false_type{}, c0, c1, by_atom);
+      // (kWideAtom compiles without the wave-per-unit walk: 105 instead of
+      // 240 VGPRs, config-2 batch 103 -> 85 ms; its points without a table
+      // block go to the kWideRest launch)
+      if constexpr (SC != kWideAtom) {
+        if (two) walk(std::true_type{}, c0, c1, by_atom);
+        else walk(std::false_type{}, c0, c1, by_atom);
+      }
     }
     return;
   }

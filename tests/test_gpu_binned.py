@@ -184,3 +184,38 @@ def test_binned_on_vertices_faces_and_edges():
     assert rc == 0
     np.testing.assert_allclose(vb[:m], ov, rtol=1e-12, atol=1e-12 * np.abs(ov).max())
     np.testing.assert_allclose(cb[:m], oc, rtol=1e-12, atol=1e-12)
+
+
+def test_wide_rows_on_tetrahedron_vertices_and_faces():
+    """A 24-atom payload with points on the vertices, edges and faces of the
+    polyhedron nodes' tetrahedra mixed with random points: many have a zero
+    slot-0 weight and no row-table block, so the binned pipeline sums them in
+    its second launch (the wave per (point, mode) kernel over the listed
+    points) while the rest go atom-major; binned equals the fused kernel bit
+    for bit in FP32 and FP64, and FP64 matches the oracle."""
+    grids, Qr, _ = S.batch_grids(n_grids=6, v_range=(800, 2000), n_q=20_000)
+    sc = [int(sum(g.tables()["vectors_elements"])) for g in grids]
+    t = grids[int(np.argmax(sc))].tables()
+    dev = capi.Grid(t)
+    verts = np.asarray(t["vertices"])
+    tets = np.asarray(t["tet_vertices"]).reshape(-1, 4)
+    r = np.random.default_rng(37)
+    pick = tets[r.integers(0, len(tets), 3000)]
+    i, j, k = pick[:, 0], pick[:, 1], pick[:, 2]
+    pts = np.vstack([verts[i], 0.5 * (verts[i] + verts[j]), (verts[i] + verts[j] + verts[k]) / 3.0])
+    B = np.asarray(t["outer_to_xyz"]).reshape(3, 3)
+    special = np.linalg.solve(B, pts.T).T
+    Q = np.vstack([special, Qr[:10_000]])
+    Q = Q[r.permutation(len(Q))]
+    say(f"wide rows on tetrahedra: {len(special)} special of {len(Q)}, {sc[int(np.argmax(sc))]} complex per mode")
+    for precision in (capi.FP32, capi.FP64):
+        rb, vb, cb = run_device(dev, Q, precision, capi.KERNEL_BINNED)
+        rt, vt, ct = run_device(dev, Q, precision, capi.KERNEL_TILE)
+        assert rb == 0 and rt == 0
+        np.testing.assert_array_equal(vb, vt)
+        np.testing.assert_array_equal(cb, ct)
+    m = 1500
+    rc, ov, oc, _, _ = O.ir_interpolate_at(t, Q[:m], nthreads=8)
+    assert rc == 0
+    assert np.max(np.abs(vb[:m] - ov)) <= 1e-12 * np.max(np.abs(ov))
+    assert np.max(np.abs(cb[:m] - oc)) <= 1e-12 * np.max(np.abs(oc))
