@@ -415,100 +415,107 @@ __global__ __launch_bounds__(TILE) void k_bin_sums(GridConst g, DataConst d, Pro
   constexpr bool kAtom = L == LAYOUT_WIDE && SC == kWideAtom;
   constexpr bool kRest = L == LAYOUT_WIDE && SC == kWideRest;
   if constexpr (kRest) n = min(n, *rest_n);
-  const uint32_t tile0 = xcd_tile(blockIdx.x, gridDim.x) * TILE;
-  if (kRest && tile0 >= n) return;  // the grid covers the largest possible list
-  const uint32_t tid = threadIdx.x;
-  const uint32_t p = tile0 + tid;
-  // wide rows: blockIdx.y picks a group of modes (gridDim.y groups; the
-  // first one reports the points' errors and aux outputs)
-  const bool lead = blockIdx.y == 0;
-  const uint32_t per_group = (d.B + gridDim.y - 1) / gridDim.y;
-  const uint32_t b_lo = blockIdx.y * per_group, b_hi = b_lo + per_group;
-  const bool want_phase = d.vec_rot == BRILLE_ROT_GAMMA && d.ec[1] > 0;
-  const uint32_t n_atoms = d.ec[1] / 3u;
-  const bool ph_in_lds = L == LAYOUT_GAMMA || (want_phase && n_atoms <= 4);
-  WideGammaLds<Real> wg;
-  if (kWide && want_phase) {
-    wg.ph = s_ph;
-    wg.rot = s_wrot;
-    wg.dst = s_wdst;
-    wg.cap = kPh;
-    wide_gamma_ops<Real, TILE>(g, d, wg);
-  }
-  __shared__ uint32_t s_wave_tab[TILE / 64];
-  if (Prop) __syncthreads();
-  const bool live = p < n;
-  SortRec sr;
-  Located P;
-  P.status = BRILLE_PT_OK;
-  P.cube = kNoCube;
-  uint32_t ridx = 0;
-  if (live) {
-    ridx = kRest ? rest[p] : (perm[p] & ~kPermPoly);
-    sr = srt[ridx];
-    P.q[0] = sr.q[0]; P.q[1] = sr.q[1]; P.q[2] = sr.q[2];
-    P.tau[0] = P.tau[1] = P.tau[2] = 0;
-    P.node = sr.node;
-    P.rot = sr.meta & 255u;
-    P.invrot = (sr.meta >> 8) & 255u;
-    P.status = (int)meta_status(sr.meta);
-    P.tet = kNoTet;
-    P.nslot = 0;
-    P.nkept = 0;
-    if (P.status == BRILLE_PT_OK) {
-      if constexpr (Prop) {
-        bool verified = false;
-        node_slots(g, P, sr.hint, pc.margin, &verified);
-        if (sr.hint != kNoTet) {
-          atomicAdd(&s_tried, 1u);
-          if (verified) atomicAdd(&s_verified, 1u);
-        }
-      } else {
-        node_slots(g, P);
-      }
-      // failed fold-ins were reported by k_bin_fold
-      if (P.status != BRILLE_PT_OK && lead) report_point(err, base + sr.index, P.status);
-      if (kAtom && lead && P.status == BRILLE_PT_OK && table_row(d, P) == kNoCube) rest[atomicAdd(rest_n, 1u)] = ridx;
+  // kWideRest: a few blocks (the launch has no count of the list) walk the
+  // listed points tile by tile; every other launch has one tile per block
+  const uint32_t first_tile = kRest ? blockIdx.x : xcd_tile(blockIdx.x, gridDim.x);
+  for (uint32_t tile = first_tile; (uint64_t)tile * TILE < n; tile += gridDim.x) {
+    if (tile != first_tile) __syncthreads();  // the previous tile's records are read
+    const uint32_t tile0 = tile * TILE;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t p = tile0 + tid;
+    // wide rows: blockIdx.y picks a group of modes (gridDim.y groups; the
+    // first one reports the points' errors and aux outputs)
+    const bool lead = blockIdx.y == 0;
+    const uint32_t per_group = (d.B + gridDim.y - 1) / gridDim.y;
+    const uint32_t b_lo = blockIdx.y * per_group, b_hi = b_lo + per_group;
+    const bool want_phase = d.vec_rot == BRILLE_ROT_GAMMA && d.ec[1] > 0;
+    const uint32_t n_atoms = d.ec[1] / 3u;
+    const bool ph_in_lds = L == LAYOUT_GAMMA || (want_phase && n_atoms <= 4);
+    WideGammaLds<Real> wg;
+    if (kWide && want_phase) {
+      wg.ph = s_ph;
+      wg.rot = s_wrot;
+      wg.dst = s_wdst;
+      wg.cap = kPh;
+      wide_gamma_ops<Real, TILE>(g, d, wg);
     }
-  }
-  // Stable partition of the tile: points the cube factor table serves first,
-  // so that few waves of the sums mix the two kinds of point.
-  const bool table = (L == LAYOUT_GAMMA || L == LAYOUT_SCALAR) && cube_rows<Real>(d) != nullptr;
-  const bool tab = table && live && P.status == BRILLE_PT_OK && table_row(d, P) != kNoCube;
-  const uint32_t lane = tid & 63u, wave = tid >> 6;
-  const unsigned long long bal = __ballot(tab);
-  const uint32_t rank = (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
-  if (lane == 0) s_wave_tab[wave] = (uint32_t)__popcll(bal);
-  __syncthreads();
-  uint32_t before = 0, n_tab = 0;
-#pragma unroll
-  for (uint32_t w = 0; w < TILE / 64; ++w) {
-    const uint32_t c = s_wave_tab[w];
-    before += w < wave ? c : 0u;
-    n_tab += c;
-  }
-  const uint32_t slot = tab ? before + rank : n_tab + (tid - before - rank);
-  {
-    PointRec<Real>& r = s_rec[slot];
-    r.nslot = 0;
+    __shared__ uint32_t s_wave_tab[TILE / 64];
+    if (Prop) __syncthreads();
+    const bool live = p < n;
+    SortRec sr;
+    Located P;
+    P.status = BRILLE_PT_OK;
+    P.cube = kNoCube;
+    uint32_t ridx = 0;
     if (live) {
-      s_out[slot] = sr.index;
-      if (P.status == BRILLE_PT_OK)
-        store_record<Real, L, SC>(g, d, P, r, s_ph, s_q, slot, ph_in_lds, n_atoms, !kOpFrames);
-      if (lead && !kRest) write_aux(aux, sr.index, P);
+      ridx = kRest ? rest[p] : (perm[p] & ~kPermPoly);
+      sr = srt[ridx];
+      P.q[0] = sr.q[0]; P.q[1] = sr.q[1]; P.q[2] = sr.q[2];
+      P.tau[0] = P.tau[1] = P.tau[2] = 0;
+      P.node = sr.node;
+      P.rot = sr.meta & 255u;
+      P.invrot = (sr.meta >> 8) & 255u;
+      P.status = (int)meta_status(sr.meta);
+      P.tet = kNoTet;
+      P.nslot = 0;
+      P.nkept = 0;
+      if (P.status == BRILLE_PT_OK) {
+        if constexpr (Prop) {
+          bool verified = false;
+          node_slots(g, P, sr.hint, pc.margin, &verified);
+          if (sr.hint != kNoTet) {
+            atomicAdd(&s_tried, 1u);
+            if (verified) atomicAdd(&s_verified, 1u);
+          }
+        } else {
+          node_slots(g, P);
+        }
+        // failed fold-ins were reported by k_bin_fold
+        if (P.status != BRILLE_PT_OK && lead) report_point(err, base + sr.index, P.status);
+        if (kAtom && lead && P.status == BRILLE_PT_OK && table_row(d, P) == kNoCube) rest[atomicAdd(rest_n, 1u)] = ridx;
+      }
     }
+    // Stable partition of the tile: points the cube factor table serves first,
+    // so that few waves of the sums mix the two kinds of point.
+    const bool table = (L == LAYOUT_GAMMA || L == LAYOUT_SCALAR) && cube_rows<Real>(d) != nullptr;
+    const bool tab = table && live && P.status == BRILLE_PT_OK && table_row(d, P) != kNoCube;
+    const uint32_t lane = tid & 63u, wave = tid >> 6;
+    const unsigned long long bal = __ballot(tab);
+    const uint32_t rank = (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
+    if (lane == 0) s_wave_tab[wave] = (uint32_t)__popcll(bal);
+    __syncthreads();
+    uint32_t before = 0, n_tab = 0;
+  #pragma unroll
+    for (uint32_t w = 0; w < TILE / 64; ++w) {
+      const uint32_t c = s_wave_tab[w];
+      before += w < wave ? c : 0u;
+      n_tab += c;
+    }
+    const uint32_t slot = tab ? before + rank : n_tab + (tid - before - rank);
+    {
+      PointRec<Real>& r = s_rec[slot];
+      r.nslot = 0;
+      if (live) {
+        s_out[slot] = sr.index;
+        if (P.status == BRILLE_PT_OK)
+          store_record<Real, L, SC>(g, d, P, r, s_ph, s_q, slot, ph_in_lds, n_atoms, !kOpFrames);
+        if (lead && !kRest) write_aux(aux, sr.index, P);
+      }
+    }
+    __syncthreads();
+    if (Prop && lead && threadIdx.x == 0 && s_tried) {
+      atomicAdd(&pc.stats[0], (unsigned long long)s_tried);
+      atomicAdd(&pc.stats[1], (unsigned long long)s_verified);
+    }
+    const uint32_t npts = n - tile0 < (uint32_t)TILE ? n - tile0 : (uint32_t)TILE;
+    // the one-lane-per-unit sums see only the points the table does not serve:
+    // fewer rows in flight keep the registers for occupancy
+    constexpr int GRP = sizeof(Real) == 4 ? 2 : 1;
+    tile_sums<Real, true, L, SC, SV, TILE, GRP>(g, d, s_rec, s_ph, s_q, s_out, 0, npts, ph_in_lds, vals, vecs, s_wide,
+                                                table, n_tab, kOpFrames ? s_frames : nullptr, true, wg, b_lo, b_hi);
+
+    if (!kRest) break;
   }
-  __syncthreads();
-  if (Prop && lead && threadIdx.x == 0 && s_tried) {
-    atomicAdd(&pc.stats[0], (unsigned long long)s_tried);
-    atomicAdd(&pc.stats[1], (unsigned long long)s_verified);
-  }
-  const uint32_t npts = n - tile0 < (uint32_t)TILE ? n - tile0 : (uint32_t)TILE;
-  // the one-lane-per-unit sums see only the points the table does not serve:
-  // fewer rows in flight keep the registers for occupancy
-  constexpr int GRP = sizeof(Real) == 4 ? 2 : 1;
-  tile_sums<Real, true, L, SC, SV, TILE, GRP>(g, d, s_rec, s_ph, s_q, s_out, 0, npts, ph_in_lds, vals, vecs, s_wide,
-                                              table, n_tab, kOpFrames ? s_frames : nullptr, true, wg, b_lo, b_hi);
 }
 
 // The row table: thread t = (block row, mode b) phases the row of a cube's or
@@ -1021,8 +1028,9 @@ int run_binned_passes(brille_grid* G, const LaunchArgs& a, int precision, hipStr
     hipLaunchKernelGGL(sums, sgrid, dim3(TILE), 0, s, G->g, G->d, a.prop, b.srt, b.perm, m,
                        static_cast<char*>(a.vals) + c0 * vb, static_cast<char*>(a.vecs) + c0 * cb,
                        offset_aux(a.aux, c0), a.err, a.base + c0, b.plist, b.pcount);
-    if (by_atom)  // the listed points without a table block
-      hipLaunchKernelGGL(rest_sums, sgrid, dim3(TILE), 0, s, G->g, G->d, a.prop, b.srt, b.perm, m,
+    if (by_atom)  // the listed points without a table block: one block per CU at most
+      hipLaunchKernelGGL(rest_sums, dim3(std::min<uint32_t>(sgrid.x, std::max<uint32_t>(G->n_cu, 1)), sgrid.y),
+                         dim3(TILE), 0, s, G->g, G->d, a.prop, b.srt, b.perm, m,
                          static_cast<char*>(a.vals) + c0 * vb, static_cast<char*>(a.vecs) + c0 * cb,
                          offset_aux(a.aux, c0), a.err, a.base + c0, b.plist, b.pcount);
     mark(k, 4, s);

@@ -862,7 +862,7 @@ constexpr int kWideAll = 0, kWideAtom = 1, kWideRest = 2;
 constexpr uint32_t kWidePhaseBytes = 36864;
 // the same for the atom-only sums (kWideAtom below), whose 109 VGPRs allow
 // four waves per SIMD when a block's LDS stays under 40 KB: 8 KB, 42 points
-// of 24 atoms per chunk (config-2 batch 85 -> 76 ms)
+// of 24 atoms per chunk (with 36 KB, two waves: about 12 % slower)
 constexpr uint32_t kWideAtomPhaseBytes = 8192;
 
 __device__ __forceinline__ float wave_sum_f(float v) {
@@ -1664,9 +1664,9 @@ __device__ __forceinline__ void tile_sums(const GridConst& g, const DataConst& d
         __syncthreads();
       }
       if (by_atom) atom_walk(c0, c1);
-      // (kWideAtom compiles without the wave-per-unit walk: 105 instead of
-      // 240 VGPRs, config-2 batch 103 -> 85 ms; its points without a table
-      // block go to the kWideRest launch)
+      // (kWideAtom compiles without the wave-per-unit walk: 109 instead of
+      // 240 VGPRs; its points without a table block go to the kWideRest
+      // launch.  Config-2 batch 103 -> 87-90 ms.)
       if constexpr (SC != kWideAtom) {
         if (two) walk(std::true_type{}, c0, c1, by_atom);
         else walk(std::false_type{}, c0, c1, by_atom);
