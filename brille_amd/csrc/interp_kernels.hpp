@@ -1479,7 +1479,7 @@ __device__ __forceinline__ void tile_sums(const GridConst& g, const DataConst& d
       // Point-major: wave w takes units w, w + TB/64, ... of (point, mode),
       // so it stays on a point for its share of the modes and keeps the
       // point's rotation data (a second mode in flight per wave doubled the
-      // registers and made the config-2 batch slower, 466 -> 511 ms)
+      // registers and made the 64-grid batch of 1e7 points slower, 466 -> 511 ms)
       uint32_t last = 0xFFFFFFFFu;
       for (uint32_t c = wave; c < units; c += TB / 64) {
         const uint32_t pl = c / B, b = c - pl * B;
@@ -1519,7 +1519,7 @@ __device__ __forceinline__ void tile_sums(const GridConst& g, const DataConst& d
     // without a table block are left to that walk.
     // (Staging a wave's rotated rows in LDS, to write them as one contiguous
     // run per point instead of three 8-byte stores per lane at a permuted
-    // atom, saved no time: config-2 batch 140 -> 145 ms.)
+    // atom, saved no time: 64-grid batch of 1e7 points 140 -> 145 ms.)
     // kWideAtom: the caller chose it for such rows; kWideRest: no point here
     // has a table block
     const bool by_atom = SC == kWideAtom || (SC == kWideAll && pre && tab && d.sv <= na);
@@ -1666,7 +1666,7 @@ __device__ __forceinline__ void tile_sums(const GridConst& g, const DataConst& d
       if (by_atom) atom_walk(c0, c1);
       // (kWideAtom compiles without the wave-per-unit walk: 109 instead of
       // 240 VGPRs; its points without a table block go to the kWideRest
-      // launch.  Config-2 batch 103 -> 87-90 ms.)
+      // launch.  64-grid batch of 1e7 points 103 -> 87-90 ms.)
       if constexpr (SC != kWideAtom) {
         if (two) walk(std::true_type{}, c0, c1, by_atom);
         else walk(std::false_type{}, c0, c1, by_atom);
